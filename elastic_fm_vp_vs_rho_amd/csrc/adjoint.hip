@@ -1,0 +1,621 @@
+// Exact discrete adjoint of the forward recurrence in forward.hip, plus the
+// model maps.  The reference has no exact adjoint (its backwardFields re-uses
+// the forward operator, M:2056-2082); these kernels are the transpose of the
+// two sweeps, derived by hand:
+//
+//   forward step n     τ' = τ + c·S(λ,μ)·e(v,ψv),  τ'' = τ' + M·w[n]δ_src,
+//                      v' = v + c·B̄(ρ)·F(τ'',ψτ),  d[n] = R v'
+//   ψ recurrences      ψ' = bψ + aΔ,  e = Δ + ψ'   (likewise for ψτ)
+//
+//   reverse step n     R1 v̄[rec] += r[n]
+//                      R2 u = c·b̄·v̄;  h̄ = u + a(C+u);  C ← b(C+u);  τ̄ += D_τᵀ h̄
+//                         ρ-accumulators  += v̄·F
+//                      R3 M̄ += w[n]·τ̄(src)
+//                      R4 ē = c·Sᵀτ̄;  ḡ = ē + a(C+ē);  C ← b(C+ē);  v̄ += D_vᵀ ḡ
+//                         λ/μ accumulators += τ̄·(∂S/∂λ,μ)e
+//
+// D_ᵀ are applied in gather form (each thread writes only its own cell), which
+// needs h̄/ḡ at the ±1 neighbours; C is read at neighbours while its owner
+// rewrites it, so C is ping-ponged between two buffers.  The derived medium
+// (μ̄, b̄) is zero outside the interior and τ̄, v̄, C are only ever written on
+// the interior, so neighbour terms vanish there without branches.
+#include <utility>
+
+#include "common.hpp"
+
+namespace efm {
+
+struct AdjArgs {
+    const float *lam, *mu;
+    const float *mbar, *bbar;      // (3, N) each
+    StatePtrs fn, fn1;             // forward S^n, S^{n+1}
+    StatePtrs a, a2;               // adjoint: a = live (v̄, τ̄, C in), a2 = C out
+    float *acc;                    // (S, 8, N)
+    ShotPtrs sh;
+};
+
+// ---------------------------------------------------------------------------
+// R2: transpose of the velocity sweep
+// ---------------------------------------------------------------------------
+template <int BX, int BZ, int NYC>
+__global__ __launch_bounds__(BX *BZ) void k_adj_velocity(Geo g, AdjArgs A, int n) {
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int z = blockIdx.y * BZ + threadIdx.y;
+    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
+    const int s = blockIdx.z / nyc;
+    const int yb = blockIdx.z - s * nyc;
+    if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
+    if (n >= A.sh.nt[s]) return;
+    const int y0 = H + yb * NYC;
+    const int y1 = min(y0 + NYC, g.ny - H);
+    const int nx = g.nx;
+    const long long sxz = g.sxz, N = g.N;
+    const long long so = (long long)s * N;
+    const float c = g.c;
+    const ShotPtrs &sh = A.sh;
+
+    // slab coordinates and coefficients of this column and its x/z neighbours
+    const int sx0 = slab_coord(x, g.nx, g.wxl, g.wxh);
+    const int sxm = slab_coord(x - 1, g.nx, g.wxl, g.wxh);
+    const int sxp = slab_coord(x + 1, g.nx, g.wxl, g.wxh);
+    const int sz0 = slab_coord(z, g.nz, g.wzl, g.wzh);
+    const int szm = slab_coord(z - 1, g.nz, g.wzl, g.wzh);
+    const int szp = slab_coord(z + 1, g.nz, g.wzl, g.wzh);
+    const float *axs = sh.ax + (long long)s * g.nx, *bxs = sh.bx + (long long)s * g.nx;
+    const float *ays = sh.ay + (long long)s * g.ny, *bys = sh.by + (long long)s * g.ny;
+    const float *azs = sh.az + (long long)s * g.nz, *bzs = sh.bz + (long long)s * g.nz;
+    const float ax0 = sx0 >= 0 ? axs[x] : 0.f, bx0 = sx0 >= 0 ? bxs[x] : 0.f;
+    const float axm = sxm >= 0 ? axs[x - 1] : 0.f, axp = sxp >= 0 ? axs[x + 1] : 0.f;
+    const float az0 = sz0 >= 0 ? azs[z] : 0.f, bz0 = sz0 >= 0 ? bzs[z] : 0.f;
+    const float azm = szm >= 0 ? azs[z - 1] : 0.f, azp = szp >= 0 ? azs[z + 1] : 0.f;
+    const long long csx = px_cstride(g), csy = py_cstride(g), csz = pz_cstride(g);
+
+    const float *bbx = A.bbar, *bby = A.bbar + N, *bbz = A.bbar + 2 * N;
+    const float *vbx = A.a.v[0], *vby = A.a.v[1], *vbz = A.a.v[2];
+
+    for (int y = y0; y < y1; ++y) {
+        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
+        const long long t = so + lc;
+        const int sy0 = slab_coord(y, g.ny, g.wyl, g.wyh);
+        const int sym = slab_coord(y - 1, g.ny, g.wyl, g.wyh);
+        const int syp = slab_coord(y + 1, g.ny, g.wyl, g.wyh);
+        const float ay0 = sy0 >= 0 ? ays[y] : 0.f, by0 = sy0 >= 0 ? bys[y] : 0.f;
+        const float aym = sym >= 0 ? ays[y - 1] : 0.f, ayp = syp >= 0 ? ays[y + 1] : 0.f;
+
+        // u = c·b̄·v̄ at p and at the neighbours the transposes need
+        const float ux = c * bbx[lc] * vbx[t];
+        const float uy = c * bby[lc] * vby[t];
+        const float uz = c * bbz[lc] * vbz[t];
+        const float ux_mx = c * bbx[lc - 1] * vbx[t - 1];        // txx_x at p-x
+        const float uy_my = c * bby[lc - sxz] * vby[t - sxz];    // tyy_y at p-y
+        const float uz_mz = c * bbz[lc - nx] * vbz[t - nx];      // tzz_z at p-z
+        const float ux_py = c * bbx[lc + sxz] * vbx[t + sxz];    // txy_y at p+y
+        const float uy_px = c * bby[lc + 1] * vby[t + 1];        // txy_x at p+x
+        const float ux_pz = c * bbx[lc + nx] * vbx[t + nx];      // txz_z at p+z
+        const float uz_px = c * bbz[lc + 1] * vbz[t + 1];        // txz_x at p+x
+        const float uy_pz = c * bby[lc + nx] * vby[t + nx];      // tyz_z at p+z
+        const float uz_py = c * bbz[lc + sxz] * vbz[t + sxz];    // tyz_y at p+y
+
+        // h̄ = u + a(C + u) with C read from the "in" buffer
+        float hb_xx_m = ux_mx, hb_xx_0 = ux;           // txx_x
+        float hb_yy_m = uy_my, hb_yy_0 = uy;           // tyy_y
+        float hb_zz_m = uz_mz, hb_zz_0 = uz;           // tzz_z
+        float hb_xyy_0 = ux, hb_xyy_p = ux_py;         // txy_y
+        float hb_xyx_0 = uy, hb_xyx_p = uy_px;         // txy_x
+        float hb_xzz_0 = ux, hb_xzz_p = ux_pz;         // txz_z
+        float hb_xzx_0 = uz, hb_xzx_p = uz_px;         // txz_x
+        float hb_yzz_0 = uy, hb_yzz_p = uy_pz;         // tyz_z
+        float hb_yzy_0 = uz, hb_yzy_p = uz_py;         // tyz_y
+        const float *Cx = A.a.px, *Cy = A.a.py, *Cz = A.a.pz;
+        float *Cxo = A.a2.px, *Cyo = A.a2.py, *Czo = A.a2.pz;
+        // x-slab components 3 txx_x, 4 txy_x, 5 txz_x
+        if (sxm >= 0) hb_xx_m += axm * (Cx[px_index(g, s, 3, y, z, sxm)] + ux_mx);
+        if (sxp >= 0) {
+            const long long pi = px_index(g, s, 4, y, z, sxp);
+            hb_xyx_p += axp * (Cx[pi] + uy_px);
+            hb_xzx_p += axp * (Cx[pi + csx] + uz_px);
+        }
+        if (sx0 >= 0) {
+            const long long pi = px_index(g, s, 3, y, z, sx0);
+            const float c0 = Cx[pi] + ux, c1 = Cx[pi + csx] + uy, c2 = Cx[pi + 2 * csx] + uz;
+            hb_xx_0 += ax0 * c0; hb_xyx_0 += ax0 * c1; hb_xzx_0 += ax0 * c2;
+            Cxo[pi] = bx0 * c0; Cxo[pi + csx] = bx0 * c1; Cxo[pi + 2 * csx] = bx0 * c2;
+        }
+        // y-slab components 3 txy_y, 4 tyy_y, 5 tyz_y
+        if (sym >= 0) hb_yy_m += aym * (Cy[py_index(g, s, 4, sym, z, x)] + uy_my);
+        if (syp >= 0) {
+            const long long pi = py_index(g, s, 3, syp, z, x);
+            hb_xyy_p += ayp * (Cy[pi] + ux_py);
+            hb_yzy_p += ayp * (Cy[pi + 2 * csy] + uz_py);
+        }
+        if (sy0 >= 0) {
+            const long long pi = py_index(g, s, 3, sy0, z, x);
+            const float c0 = Cy[pi] + ux, c1 = Cy[pi + csy] + uy, c2 = Cy[pi + 2 * csy] + uz;
+            hb_xyy_0 += ay0 * c0; hb_yy_0 += ay0 * c1; hb_yzy_0 += ay0 * c2;
+            Cyo[pi] = by0 * c0; Cyo[pi + csy] = by0 * c1; Cyo[pi + 2 * csy] = by0 * c2;
+        }
+        // z-slab components 3 txz_z, 4 tyz_z, 5 tzz_z
+        if (szm >= 0) hb_zz_m += azm * (Cz[pz_index(g, s, 5, y, szm, x)] + uz_mz);
+        if (szp >= 0) {
+            const long long pi = pz_index(g, s, 3, y, szp, x);
+            hb_xzz_p += azp * (Cz[pi] + ux_pz);
+            hb_yzz_p += azp * (Cz[pi + csz] + uy_pz);
+        }
+        if (sz0 >= 0) {
+            const long long pi = pz_index(g, s, 3, y, sz0, x);
+            const float c0 = Cz[pi] + ux, c1 = Cz[pi + csz] + uy, c2 = Cz[pi + 2 * csz] + uz;
+            hb_xzz_0 += az0 * c0; hb_yzz_0 += az0 * c1; hb_zz_0 += az0 * c2;
+            Czo[pi] = bz0 * c0; Czo[pi + csz] = bz0 * c1; Czo[pi + 2 * csz] = bz0 * c2;
+        }
+        // τ̄ += D_τᵀ h̄
+        A.a.t[0][t] += hb_xx_m - hb_xx_0;
+        A.a.t[1][t] += hb_yy_m - hb_yy_0;
+        A.a.t[2][t] += hb_zz_m - hb_zz_0;
+        A.a.t[3][t] += (hb_xyy_0 - hb_xyy_p) + (hb_xyx_0 - hb_xyx_p);
+        A.a.t[4][t] += (hb_xzz_0 - hb_xzz_p) + (hb_xzx_0 - hb_xzx_p);
+        A.a.t[5][t] += (hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p);
+
+        // ρ accumulators: forward F at p from τ^{n+1} (= τ'' of step n) and ψτ^n
+        const float *T0 = A.fn1.t[0], *T1 = A.fn1.t[1], *T2 = A.fn1.t[2];
+        const float *T3 = A.fn1.t[3], *T4 = A.fn1.t[4], *T5 = A.fn1.t[5];
+        const float h_xx = T0[t + 1] - T0[t];
+        const float h_yy = T1[t + sxz] - T1[t];
+        const float h_zz = T2[t + nx] - T2[t];
+        const float h_xyy = T3[t] - T3[t - sxz], h_xyx = T3[t] - T3[t - 1];
+        const float h_xzz = T4[t] - T4[t - nx], h_xzx = T4[t] - T4[t - 1];
+        const float h_yzz = T5[t] - T5[t - nx], h_yzy = T5[t] - T5[t - sxz];
+        float f_xx = h_xx, f_yy = h_yy, f_zz = h_zz, f_xyy = h_xyy, f_xyx = h_xyx;
+        float f_xzz = h_xzz, f_xzx = h_xzx, f_yzz = h_yzz, f_yzy = h_yzy;
+        if (sx0 >= 0) {
+            const long long pi = px_index(g, s, 3, y, z, sx0);
+            f_xx += bx0 * A.fn.px[pi] + ax0 * h_xx;
+            f_xyx += bx0 * A.fn.px[pi + csx] + ax0 * h_xyx;
+            f_xzx += bx0 * A.fn.px[pi + 2 * csx] + ax0 * h_xzx;
+        }
+        if (sy0 >= 0) {
+            const long long pi = py_index(g, s, 3, sy0, z, x);
+            f_xyy += by0 * A.fn.py[pi] + ay0 * h_xyy;
+            f_yy += by0 * A.fn.py[pi + csy] + ay0 * h_yy;
+            f_yzy += by0 * A.fn.py[pi + 2 * csy] + ay0 * h_yzy;
+        }
+        if (sz0 >= 0) {
+            const long long pi = pz_index(g, s, 3, y, sz0, x);
+            f_xzz += bz0 * A.fn.pz[pi] + az0 * h_xzz;
+            f_yzz += bz0 * A.fn.pz[pi + csz] + az0 * h_yzz;
+            f_zz += bz0 * A.fn.pz[pi + 2 * csz] + az0 * h_zz;
+        }
+        float *acc = A.acc + (long long)s * 8 * N + lc;
+        acc[5 * N] += vbx[t] * (f_xx + f_xyy + f_xzz);
+        acc[6 * N] += vby[t] * (f_xyx + f_yy + f_yzz);
+        acc[7 * N] += vbz[t] * (f_xzx + f_yzy + f_zz);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// R4: transpose of the stress sweep
+// ---------------------------------------------------------------------------
+template <int BX, int BZ, int NYC>
+__global__ __launch_bounds__(BX *BZ) void k_adj_stress(Geo g, AdjArgs A, int n) {
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int z = blockIdx.y * BZ + threadIdx.y;
+    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
+    const int s = blockIdx.z / nyc;
+    const int yb = blockIdx.z - s * nyc;
+    if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
+    if (n >= A.sh.nt[s]) return;
+    const int y0 = H + yb * NYC;
+    const int y1 = min(y0 + NYC, g.ny - H);
+    const int nx = g.nx;
+    const long long sxz = g.sxz, N = g.N;
+    const long long so = (long long)s * N;
+    const float c = g.c;
+    const ShotPtrs &sh = A.sh;
+
+    const int sx0 = slab_coord(x, g.nx, g.wxl, g.wxh);
+    const int sxm = slab_coord(x - 1, g.nx, g.wxl, g.wxh);
+    const int sxp = slab_coord(x + 1, g.nx, g.wxl, g.wxh);
+    const int sz0 = slab_coord(z, g.nz, g.wzl, g.wzh);
+    const int szm = slab_coord(z - 1, g.nz, g.wzl, g.wzh);
+    const int szp = slab_coord(z + 1, g.nz, g.wzl, g.wzh);
+    const float *axs = sh.ax + (long long)s * g.nx, *bxs = sh.bx + (long long)s * g.nx;
+    const float *ays = sh.ay + (long long)s * g.ny, *bys = sh.by + (long long)s * g.ny;
+    const float *azs = sh.az + (long long)s * g.nz, *bzs = sh.bz + (long long)s * g.nz;
+    const float ax0 = sx0 >= 0 ? axs[x] : 0.f, bx0 = sx0 >= 0 ? bxs[x] : 0.f;
+    const float axm = sxm >= 0 ? axs[x - 1] : 0.f, axp = sxp >= 0 ? axs[x + 1] : 0.f;
+    const float az0 = sz0 >= 0 ? azs[z] : 0.f, bz0 = sz0 >= 0 ? bzs[z] : 0.f;
+    const float azm = szm >= 0 ? azs[z - 1] : 0.f, azp = szp >= 0 ? azs[z + 1] : 0.f;
+    const long long csx = px_cstride(g), csy = py_cstride(g), csz = pz_cstride(g);
+
+    const float *mxy = A.mbar, *mxz = A.mbar + N, *myz = A.mbar + 2 * N;
+    const float *lam = A.lam, *mu = A.mu;
+    const float *Txx = A.a.t[0], *Tyy = A.a.t[1], *Tzz = A.a.t[2];
+    const float *Txy = A.a.t[3], *Txz = A.a.t[4], *Tyz = A.a.t[5];
+
+    for (int y = y0; y < y1; ++y) {
+        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
+        const long long t = so + lc;
+        const int sy0 = slab_coord(y, g.ny, g.wyl, g.wyh);
+        const int sym = slab_coord(y - 1, g.ny, g.wyl, g.wyh);
+        const int syp = slab_coord(y + 1, g.ny, g.wyl, g.wyh);
+        const float ay0 = sy0 >= 0 ? ays[y] : 0.f, by0 = sy0 >= 0 ? bys[y] : 0.f;
+        const float aym = sym >= 0 ? ays[y - 1] : 0.f, ayp = syp >= 0 ? ays[y + 1] : 0.f;
+
+        const float txx = Txx[t], tyy = Tyy[t], tzz = Tzz[t];
+        const float txy = Txy[t], txz = Txz[t], tyz = Tyz[t];
+
+        // ē at p
+        const float L0 = lam[lc], M0 = mu[lc];
+        const float e_xx = c * ((L0 + 2.f * M0) * txx + L0 * (tyy + tzz));
+        const float e_yy = c * ((L0 + 2.f * M0) * tyy + L0 * (txx + tzz));
+        const float e_zz = c * ((L0 + 2.f * M0) * tzz + L0 * (txx + tyy));
+        const float e_xy = c * mxy[lc] * txy;
+        const float e_xz = c * mxz[lc] * txz;
+        const float e_yz = c * myz[lc] * tyz;
+        // ē at the neighbours
+        auto normal_xx = [&](long long o) {
+            const float L = lam[lc + o], M = mu[lc + o];
+            return c * ((L + 2.f * M) * Txx[t + o] + L * (Tyy[t + o] + Tzz[t + o]));
+        };
+        auto normal_yy = [&](long long o) {
+            const float L = lam[lc + o], M = mu[lc + o];
+            return c * ((L + 2.f * M) * Tyy[t + o] + L * (Txx[t + o] + Tzz[t + o]));
+        };
+        auto normal_zz = [&](long long o) {
+            const float L = lam[lc + o], M = mu[lc + o];
+            return c * ((L + 2.f * M) * Tzz[t + o] + L * (Txx[t + o] + Tyy[t + o]));
+        };
+        const float exx_px = normal_xx(1);                       // vx_x at p+x
+        const float eyy_py = normal_yy(sxz);                     // vy_y at p+y
+        const float ezz_pz = normal_zz(nx);                      // vz_z at p+z
+        const float exz_mz = c * mxz[lc - nx] * Txz[t - nx];     // vx_z at p-z
+        const float exy_my = c * mxy[lc - sxz] * Txy[t - sxz];   // vx_y at p-y
+        const float exy_mx = c * mxy[lc - 1] * Txy[t - 1];       // vy_x at p-x
+        const float eyz_mz = c * myz[lc - nx] * Tyz[t - nx];     // vy_z at p-z
+        const float exz_mx = c * mxz[lc - 1] * Txz[t - 1];       // vz_x at p-x
+        const float eyz_my = c * myz[lc - sxz] * Tyz[t - sxz];   // vz_y at p-y
+
+        float g_vxx_0 = e_xx, g_vxx_p = exx_px;     // vx_x
+        float g_vyy_0 = e_yy, g_vyy_p = eyy_py;     // vy_y
+        float g_vzz_0 = e_zz, g_vzz_p = ezz_pz;     // vz_z
+        float g_vxz_m = exz_mz, g_vxz_0 = e_xz;     // vx_z
+        float g_vxy_m = exy_my, g_vxy_0 = e_xy;     // vx_y
+        float g_vyx_m = exy_mx, g_vyx_0 = e_xy;     // vy_x
+        float g_vyz_m = eyz_mz, g_vyz_0 = e_yz;     // vy_z
+        float g_vzx_m = exz_mx, g_vzx_0 = e_xz;     // vz_x
+        float g_vzy_m = eyz_my, g_vzy_0 = e_yz;     // vz_y
+        const float *Cx = A.a.px, *Cy = A.a.py, *Cz = A.a.pz;
+        float *Cxo = A.a2.px, *Cyo = A.a2.py, *Czo = A.a2.pz;
+        // x-slab components 0 vx_x, 1 vy_x, 2 vz_x
+        if (sxp >= 0) g_vxx_p += axp * (Cx[px_index(g, s, 0, y, z, sxp)] + exx_px);
+        if (sxm >= 0) {
+            const long long pi = px_index(g, s, 1, y, z, sxm);
+            g_vyx_m += axm * (Cx[pi] + exy_mx);
+            g_vzx_m += axm * (Cx[pi + csx] + exz_mx);
+        }
+        if (sx0 >= 0) {
+            const long long pi = px_index(g, s, 0, y, z, sx0);
+            const float c0 = Cx[pi] + e_xx, c1 = Cx[pi + csx] + e_xy, c2 = Cx[pi + 2 * csx] + e_xz;
+            g_vxx_0 += ax0 * c0; g_vyx_0 += ax0 * c1; g_vzx_0 += ax0 * c2;
+            Cxo[pi] = bx0 * c0; Cxo[pi + csx] = bx0 * c1; Cxo[pi + 2 * csx] = bx0 * c2;
+        }
+        // y-slab components 0 vx_y, 1 vy_y, 2 vz_y
+        if (syp >= 0) g_vyy_p += ayp * (Cy[py_index(g, s, 1, syp, z, x)] + eyy_py);
+        if (sym >= 0) {
+            const long long pi = py_index(g, s, 0, sym, z, x);
+            g_vxy_m += aym * (Cy[pi] + exy_my);
+            g_vzy_m += aym * (Cy[pi + 2 * csy] + eyz_my);
+        }
+        if (sy0 >= 0) {
+            const long long pi = py_index(g, s, 0, sy0, z, x);
+            const float c0 = Cy[pi] + e_xy, c1 = Cy[pi + csy] + e_yy, c2 = Cy[pi + 2 * csy] + e_yz;
+            g_vxy_0 += ay0 * c0; g_vyy_0 += ay0 * c1; g_vzy_0 += ay0 * c2;
+            Cyo[pi] = by0 * c0; Cyo[pi + csy] = by0 * c1; Cyo[pi + 2 * csy] = by0 * c2;
+        }
+        // z-slab components 0 vx_z, 1 vy_z, 2 vz_z
+        if (szp >= 0) g_vzz_p += azp * (Cz[pz_index(g, s, 2, y, szp, x)] + ezz_pz);
+        if (szm >= 0) {
+            const long long pi = pz_index(g, s, 0, y, szm, x);
+            g_vxz_m += azm * (Cz[pi] + exz_mz);
+            g_vyz_m += azm * (Cz[pi + csz] + eyz_mz);
+        }
+        if (sz0 >= 0) {
+            const long long pi = pz_index(g, s, 0, y, sz0, x);
+            const float c0 = Cz[pi] + e_xz, c1 = Cz[pi + csz] + e_yz, c2 = Cz[pi + 2 * csz] + e_zz;
+            g_vxz_0 += az0 * c0; g_vyz_0 += az0 * c1; g_vzz_0 += az0 * c2;
+            Czo[pi] = bz0 * c0; Czo[pi + csz] = bz0 * c1; Czo[pi + 2 * csz] = bz0 * c2;
+        }
+        // v̄ += D_vᵀ ḡ
+        A.a.v[0][t] += (g_vxx_0 - g_vxx_p) + (g_vxz_m - g_vxz_0) + (g_vxy_m - g_vxy_0);
+        A.a.v[1][t] += (g_vyy_0 - g_vyy_p) + (g_vyx_m - g_vyx_0) + (g_vyz_m - g_vyz_0);
+        A.a.v[2][t] += (g_vzz_0 - g_vzz_p) + (g_vzx_m - g_vzx_0) + (g_vzy_m - g_vzy_0);
+
+        // λ, μ accumulators: forward e at p from v^n and ψv^n
+        const float *V0 = A.fn.v[0], *V1 = A.fn.v[1], *V2 = A.fn.v[2];
+        const float v0 = V0[t], v1 = V1[t], v2 = V2[t];
+        const float d_xx = v0 - V0[t - 1], d_yy = v1 - V1[t - sxz], d_zz = v2 - V2[t - nx];
+        const float d_xz = V0[t + nx] - v0, d_xy = V0[t + sxz] - v0;
+        const float d_yx = V1[t + 1] - v1, d_yz = V1[t + nx] - v1;
+        const float d_zx = V2[t + 1] - v2, d_zy = V2[t + sxz] - v2;
+        float fxx = d_xx, fyy = d_yy, fzz = d_zz, fxz = d_xz, fxy = d_xy;
+        float fyx = d_yx, fyz = d_yz, fzx = d_zx, fzy = d_zy;
+        if (sx0 >= 0) {
+            const long long pi = px_index(g, s, 0, y, z, sx0);
+            fxx += bx0 * A.fn.px[pi] + ax0 * d_xx;
+            fyx += bx0 * A.fn.px[pi + csx] + ax0 * d_yx;
+            fzx += bx0 * A.fn.px[pi + 2 * csx] + ax0 * d_zx;
+        }
+        if (sy0 >= 0) {
+            const long long pi = py_index(g, s, 0, sy0, z, x);
+            fxy += by0 * A.fn.py[pi] + ay0 * d_xy;
+            fyy += by0 * A.fn.py[pi + csy] + ay0 * d_yy;
+            fzy += by0 * A.fn.py[pi + 2 * csy] + ay0 * d_zy;
+        }
+        if (sz0 >= 0) {
+            const long long pi = pz_index(g, s, 0, y, sz0, x);
+            fxz += bz0 * A.fn.pz[pi] + az0 * d_xz;
+            fyz += bz0 * A.fn.pz[pi + csz] + az0 * d_yz;
+            fzz += bz0 * A.fn.pz[pi + 2 * csz] + az0 * d_zz;
+        }
+        float *acc = A.acc + (long long)s * 8 * N + lc;
+        acc[0] += (txx + tyy + tzz) * (fxx + fyy + fzz);
+        acc[N] += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
+        acc[2 * N] += txy * (fxy + fyx);
+        acc[3 * N] += txz * (fxz + fzx);
+        acc[4 * N] += tyz * (fyz + fzy);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// R1 + R3 (one block per shot): moment-tensor sum of step n+1, residual of step n
+// ---------------------------------------------------------------------------
+__global__ void k_adj_points(Geo g, ShotPtrs sh, StatePtrs a, const float *resid, double *gm6,
+                             int n, int n_m) {
+    const int s = blockIdx.x;
+    const int nts = sh.nt[s];
+    if (threadIdx.x == 0 && n_m >= 0 && n_m < nts) {
+        const int *sp = sh.src + 3 * s;
+        const long long t = (long long)s * g.N + sp[0] + (long long)sp[2] * g.nx + (long long)sp[1] * g.sxz;
+        const double w = sh.wav[(long long)s * g.ntmax + n_m];
+        for (int k = 0; k < 6; ++k) gm6[6 * s + k] += w * (double)a.t[k][t];
+    }
+    if (n < 0 || n >= nts) return;
+    for (int r = threadIdx.x; r < g.nrec; r += blockDim.x) {
+        const int cell = sh.rec[(long long)s * g.nrec + r];
+        if (cell < 0) continue;
+        const long long t = (long long)s * g.N + cell;
+        const long long base = ((long long)s * 3 * g.ntmax + n) * g.nrec + r;
+        const long long cs = (long long)g.ntmax * g.nrec;
+        atomicAdd(&a.v[0][t], resid[base]);
+        atomicAdd(&a.v[1][t], resid[base + cs]);
+        atomicAdd(&a.v[2][t], resid[base + 2 * cs]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// derived medium, gradient combine, model maps
+// ---------------------------------------------------------------------------
+__global__ void k_derived(Geo g, const float *mu, const float *rho, float *mbar, float *bbar) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    const int x = (int)(i % g.nx);
+    const int z = (int)((i / g.nx) % g.nz);
+    const int y = (int)(i / g.sxz);
+    float mxy = 0.f, mxz = 0.f, myz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+    if (interior(x, y, z, g)) {
+        const long long nx = g.nx, sxz = g.sxz;
+        const float q0 = 0.25f / mu[i], qi = 0.25f / mu[i + 1], qk = 0.25f / mu[i + nx];
+        const float qj = 0.25f / mu[i + sxz], qik = 0.25f / mu[i + 1 + nx];
+        const float qjk = 0.25f / mu[i + nx + sxz], qij = 0.25f / mu[i + 1 + sxz];
+        mxz = 1.f / (q0 + qi + qk + qik);
+        mxy = 1.f / (q0 + qi + qj + qij);
+        myz = 1.f / (q0 + qj + qk + qjk);
+        const float r = rho[i];
+        bx = 2.f / (r + rho[i + 1]);
+        by = 2.f / (r + rho[i + sxz]);
+        bz = 2.f / (r + rho[i + nx]);
+    }
+    mbar[i] = mxy; mbar[i + g.N] = mxz; mbar[i + 2 * g.N] = myz;
+    bbar[i] = bx; bbar[i + g.N] = by; bbar[i + 2 * g.N] = bz;
+}
+
+__global__ void k_combine(Geo g, const float *mu, const float *mbar, const float *bbar,
+                          const float *acc, float *glam, float *gmu, float *grho) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    const int x = (int)(i % g.nx);
+    const int z = (int)((i / g.nx) % g.nz);
+    const int y = (int)(i / g.sxz);
+    const long long N = g.N, nx = g.nx, sxz = g.sxz;
+    const float c = g.c;
+    const float *mxy = mbar, *mxz = mbar + N, *myz = mbar + 2 * N;
+    const float *bbx = bbar, *bby = bbar + N, *bbz = bbar + 2 * N;
+    float sl = 0.f, sm = 0.f, shear = 0.f, sr = 0.f;
+    for (int s = 0; s < g.S; ++s) {
+        const float *A = acc + (long long)s * 8 * N;
+        sl += A[i];
+        sm += A[N + i];
+        auto sh4 = [&](int comp, const float *mb, long long o1, bool v1, long long o2, bool v2) {
+            float r = A[comp * N + i] * mb[i] * mb[i];
+            if (v1) r += A[comp * N + i - o1] * mb[i - o1] * mb[i - o1];
+            if (v2) r += A[comp * N + i - o2] * mb[i - o2] * mb[i - o2];
+            if (v1 && v2) r += A[comp * N + i - o1 - o2] * mb[i - o1 - o2] * mb[i - o1 - o2];
+            return r;
+        };
+        shear += sh4(2, mxy, 1, x > 0, sxz, y > 0);
+        shear += sh4(3, mxz, 1, x > 0, nx, z > 0);
+        shear += sh4(4, myz, sxz, y > 0, nx, z > 0);
+        float r = A[5 * N + i] * bbx[i] * bbx[i] + A[6 * N + i] * bby[i] * bby[i]
+                + A[7 * N + i] * bbz[i] * bbz[i];
+        if (x > 0) r += A[5 * N + i - 1] * bbx[i - 1] * bbx[i - 1];
+        if (y > 0) r += A[6 * N + i - sxz] * bby[i - sxz] * bby[i - sxz];
+        if (z > 0) r += A[7 * N + i - nx] * bbz[i - nx] * bbz[i - nx];
+        sr += r;
+    }
+    const float m = mu[i];
+    glam[i] = c * sl;
+    gmu[i] = c * (sm + (m != 0.f ? 0.25f * shear / (m * m) : 0.f));
+    grho[i] = -0.5f * c * sr;
+}
+
+__global__ void k_lame(const float *vp, const float *vs, const float *rho, float *lam, float *mu,
+                       long long n) {
+#pragma clang fp contract(off)
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float m = vs[i] * vs[i] * rho[i];
+    mu[i] = m;
+    lam[i] = vp[i] * vp[i] * rho[i] - 2.0f * m;
+}
+
+__global__ void k_lame_adjoint(const float *vp, const float *vs, const float *rho,
+                               const float *glam, const float *gmu, const float *grho_in,
+                               float *gvp, float *gvs, float *grho_out, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float p = vp[i], s = vs[i], r = rho[i], gl = glam[i], gm = gmu[i];
+    gvp[i] = 2.f * r * p * gl;
+    gvs[i] = 2.f * r * s * (gm - 2.f * gl);
+    grho_out[i] = (p * p - 2.f * s * s) * gl + s * s * gm + (grho_in ? grho_in[i] : 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define EFM_TILES(X)  \
+    X(64, 4, 1)       \
+    X(64, 4, 4)       \
+    X(64, 4, 8)       \
+    X(64, 2, 8)       \
+    X(64, 8, 4)       \
+    X(64, 4, 16)      \
+    X(128, 2, 4)      \
+    X(64, 1, 16)
+
+static inline dim3 sweep_grid(const Geo &g, const Tile &t) {
+    const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
+    return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
+}
+
+#define LAUNCH_IF(BX_, BZ_, NY_, KER, ...)                                                     \
+    if (!launched && t.bx == BX_ && t.bz == BZ_ && t.ny == NY_) {                               \
+        hipLaunchKernelGGL((KER<BX_, BZ_, NY_>), sweep_grid(g, t), dim3(BX_, BZ_), 0, st,       \
+                           __VA_ARGS__);                                                        \
+        launched = true;                                                                       \
+    }
+
+int launch_adj_velocity(const Geo &g, const AdjArgs &A, int n, hipStream_t st) {
+    const Tile t = get_tile(2);
+    bool launched = false;
+#define X(a, b, c) LAUNCH_IF(a, b, c, k_adj_velocity, g, A, n)
+    EFM_TILES(X)
+#undef X
+    if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported adjoint-velocity tile");
+    return check_launch("adjoint velocity sweep");
+}
+
+int launch_adj_stress(const Geo &g, const AdjArgs &A, int n, hipStream_t st) {
+    const Tile t = get_tile(3);
+    bool launched = false;
+#define X(a, b, c) LAUNCH_IF(a, b, c, k_adj_stress, g, A, n)
+    EFM_TILES(X)
+#undef X
+    if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported adjoint-stress tile");
+    return check_launch("adjoint stress sweep");
+}
+
+}  // namespace efm
+
+using namespace efm;
+
+static inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float *bbar,
+                                  void *stream) {
+    if (!d || !m || !mbar || !bbar) return fail(EFM_E_ARG, "efm_derived_medium: null argument");
+    const Geo g = make_geo(*d);
+    hipLaunchKernelGGL(k_derived, dim3(blocks_for(g.N, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                       m->mu, m->rho, mbar, bbar);
+    return check_launch("derived medium");
+}
+
+extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mbar,
+                           const float *bbar, const EfmShots *sh, const EfmState *const *fwd,
+                           const EfmState *adj, const EfmState *adj2, const float *resid,
+                           float *acc, double *gm6, int n0, int n1, int last, void *stream) {
+    if (!d || !m || !mbar || !bbar || !sh || !fwd || !adj || !adj2 || !resid || !acc || !gm6)
+        return fail(EFM_E_ARG, "efm_adjoint: null argument");
+    if (n0 < 0 || n1 <= n0 || n1 > d->ntmax) return fail(EFM_E_ARG, "efm_adjoint: bad step range");
+    const Geo g = make_geo(*d);
+    hipStream_t st = (hipStream_t)stream;
+    AdjArgs A;
+    A.lam = m->lam; A.mu = m->mu; A.mbar = mbar; A.bbar = bbar; A.acc = acc;
+    A.sh = make_shots(*sh);
+    StatePtrs live = make_state(*adj), other = make_state(*adj2);
+    for (int n = n1 - 1; n >= n0; --n) {
+        hipLaunchKernelGGL(k_adj_points, dim3(g.S), dim3(64), 0, st, g, A.sh, live, resid, gm6, n,
+                           n + 1 < d->ntmax ? n + 1 : -1);
+        int rc = check_launch("adjoint points");
+        if (rc) return rc;
+        A.fn = make_state(*fwd[n - n0]);
+        A.fn1 = make_state(*fwd[n - n0 + 1]);
+        A.a = live;
+        A.a2 = other;
+        A.a2.v[0] = live.v[0]; A.a2.v[1] = live.v[1]; A.a2.v[2] = live.v[2];
+        for (int k = 0; k < 6; ++k) A.a2.t[k] = live.t[k];
+        if ((rc = launch_adj_velocity(g, A, n, st))) return rc;
+        // R4 reads the τ̄-side C from `other` (just written) and its own v̄-side C from `live`:
+        // the two sweeps touch disjoint ψ components, so swap only after both.
+        if ((rc = launch_adj_stress(g, A, n, st))) return rc;
+        std::swap(live.px, other.px); std::swap(live.py, other.py); std::swap(live.pz, other.pz);
+    }
+    if (last && n0 == 0) {
+        hipLaunchKernelGGL(k_adj_points, dim3(g.S), dim3(64), 0, st, g, A.sh, live, resid, gm6, -1, 0);
+        int rc = check_launch("adjoint points");
+        if (rc) return rc;
+    }
+    // leave the live ψ adjoint in adj's buffers (copy back when they were swapped)
+    if (live.px != adj->psi_x) {
+        const long long fx = (long long)g.S * 6 * g.ny * g.nz * g.Lx;
+        const long long fy = (long long)g.S * 6 * g.Ly * g.nz * g.nx;
+        const long long fz = (long long)g.S * 6 * g.ny * g.Lz * g.nx;
+        if (fx && hipMemcpyAsync(adj->psi_x, live.px, fx * 4, hipMemcpyDeviceToDevice, st))
+            return fail(EFM_E_HIP, "efm_adjoint: psi copy");
+        if (fy && hipMemcpyAsync(adj->psi_y, live.py, fy * 4, hipMemcpyDeviceToDevice, st))
+            return fail(EFM_E_HIP, "efm_adjoint: psi copy");
+        if (fz && hipMemcpyAsync(adj->psi_z, live.pz, fz * 4, hipMemcpyDeviceToDevice, st))
+            return fail(EFM_E_HIP, "efm_adjoint: psi copy");
+    }
+    return EFM_OK;
+}
+
+extern "C" int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const float *mbar,
+                                    const float *bbar, const float *acc, float *glam, float *gmu,
+                                    float *grho, void *stream) {
+    if (!d || !m || !mbar || !bbar || !acc || !glam || !gmu || !grho)
+        return fail(EFM_E_ARG, "efm_gradient_combine: null argument");
+    const Geo g = make_geo(*d);
+    hipLaunchKernelGGL(k_combine, dim3(blocks_for(g.N, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                       m->mu, mbar, bbar, acc, glam, gmu, grho);
+    return check_launch("gradient combine");
+}
+
+extern "C" int efm_model_lame(const float *vp, const float *vs, const float *rho, float *lam,
+                              float *mu, long long n, void *stream) {
+    if (!vp || !vs || !rho || !lam || !mu || n < 0) return fail(EFM_E_ARG, "efm_model_lame: bad argument");
+    if (n == 0) return EFM_OK;
+    hipLaunchKernelGGL(k_lame, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, vp, vs,
+                       rho, lam, mu, n);
+    return check_launch("model lame");
+}
+
+extern "C" int efm_model_lame_adjoint(const float *vp, const float *vs, const float *rho,
+                                      const float *glam, const float *gmu, const float *grho_in,
+                                      float *gvp, float *gvs, float *grho_out, long long n,
+                                      void *stream) {
+    if (!vp || !vs || !rho || !glam || !gmu || !gvp || !gvs || !grho_out || n < 0)
+        return fail(EFM_E_ARG, "efm_model_lame_adjoint: bad argument");
+    if (n == 0) return EFM_OK;
+    hipLaunchKernelGGL(k_lame_adjoint, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       vp, vs, rho, glam, gmu, grho_in, gvp, gvs, grho_out, n);
+    return check_launch("model lame adjoint");
+}

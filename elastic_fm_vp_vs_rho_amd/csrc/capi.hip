@@ -1,0 +1,176 @@
+// extern "C" entry points of libefm.so that are not defined next to their
+// kernels, plus error plumbing and the tile registry used by the autotuner.
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "common.hpp"
+
+namespace efm {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(EFM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return EFM_OK;
+}
+
+// Tiles: 0 stress, 1 velocity, 2 adjoint velocity, 3 adjoint stress.
+static std::mutex g_tile_mu;
+static Tile g_tiles[4] = {{64, 4, 4}, {64, 4, 4}, {64, 4, 4}, {64, 4, 4}};
+static const Tile kTiles[] = {{64, 4, 1}, {64, 4, 4}, {64, 4, 8}, {64, 2, 8},
+                              {64, 8, 4}, {64, 4, 16}, {128, 2, 4}, {64, 1, 16}};
+
+Tile get_tile(int kernel) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    return g_tiles[kernel];
+}
+
+int set_tile(int kernel, Tile t) {
+    if (kernel < 0 || kernel > 3) return fail(EFM_E_ARG, "tile kernel id out of range");
+    for (const Tile &k : kTiles)
+        if (k.bx == t.bx && k.bz == t.bz && k.ny == t.ny) {
+            std::lock_guard<std::mutex> lk(g_tile_mu);
+            g_tiles[kernel] = t;
+            return EFM_OK;
+        }
+    return fail(EFM_E_UNSUPPORTED, "tile not compiled into this library");
+}
+
+int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                      const StatePtrs &out, int it, int src_mode, hipStream_t st);
+
+static int validate(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
+    if (!d || !m || !sh) return fail(EFM_E_ARG, "null argument");
+    if (d->nx < 5 || d->ny < 5 || d->nz < 5 || d->nshot < 1)
+        return fail(EFM_E_ARG, "grid must be at least 5 cells per axis and nshot >= 1");
+    if (d->wx_lo < 0 || d->wx_hi < 0 || d->wy_lo < 0 || d->wy_hi < 0 || d->wz_lo < 0 ||
+        d->wz_hi < 0 || d->wx_lo + d->wx_hi > d->nx || d->wy_lo + d->wy_hi > d->ny ||
+        d->wz_lo + d->wz_hi > d->nz)
+        return fail(EFM_E_ARG, "CPML widths out of range");
+    if (d->ntmax < 1 || d->nrec < 0) return fail(EFM_E_ARG, "ntmax must be >= 1 and nrec >= 0");
+    if (!m->lam || !m->mu || !m->rho) return fail(EFM_E_ARG, "medium pointer is null");
+    if (!sh->ax || !sh->bx || !sh->ay || !sh->by || !sh->az || !sh->bz || !sh->src || !sh->m6 ||
+        !sh->wavelet || !sh->nt || (d->nrec > 0 && !sh->rec))
+        return fail(EFM_E_ARG, "shot pointer is null");
+    return EFM_OK;
+}
+
+static int validate_state(const EfmState *s) {
+    if (!s || !s->vx || !s->vy || !s->vz || !s->txx || !s->tyy || !s->tzz || !s->txy ||
+        !s->txz || !s->tyz)
+        return fail(EFM_E_ARG, "state pointer is null");
+    return EFM_OK;
+}
+
+}  // namespace efm
+
+using namespace efm;
+
+extern "C" int efm_version(void) { return 10000; }  // 1.0.0
+
+extern "C" int efm_last_error(char *buf, int len) {
+    if (!buf || len <= 0) return EFM_E_ARG;
+    std::strncpy(buf, g_err.c_str(), (size_t)len - 1);
+    buf[len - 1] = '\0';
+    return (int)g_err.size();
+}
+
+extern "C" int efm_set_tile(int kernel, int bx, int bz, int ny) {
+    return set_tile(kernel, Tile{bx, bz, ny});
+}
+
+extern "C" int efm_get_tile(int kernel, int *bx, int *bz, int *ny) {
+    if (kernel < 0 || kernel > 3 || !bx || !bz || !ny) return fail(EFM_E_ARG, "efm_get_tile: bad argument");
+    const Tile t = get_tile(kernel);
+    *bx = t.bx; *bz = t.bz; *ny = t.ny;
+    return EFM_OK;
+}
+
+extern "C" int efm_tile_variants(int *out, int max) {
+    const int n = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
+    if (out)
+        for (int i = 0; i < n && i < max; ++i) {
+            out[3 * i] = kTiles[i].bx; out[3 * i + 1] = kTiles[i].bz; out[3 * i + 2] = kTiles[i].ny;
+        }
+    return n;
+}
+
+extern "C" int efm_stress_step(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                               const EfmState *in, const EfmState *out, int it, void *stream) {
+    int rc = validate(d, m, sh);
+    if (rc || (rc = validate_state(in)) || (rc = validate_state(out))) return rc;
+    return launch_stress_src(make_geo(*d), *m, make_shots(*sh), make_state(*in), make_state(*out),
+                             it, 1, (hipStream_t)stream);
+}
+
+extern "C" int efm_velocity_step(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                                 const EfmState *in, const EfmState *out, int it, void *stream) {
+    int rc = validate(d, m, sh);
+    if (rc || (rc = validate_state(in)) || (rc = validate_state(out))) return rc;
+    return launch_velocity(make_geo(*d), *m, make_shots(*sh), make_state(*in), make_state(*out), it,
+                           (hipStream_t)stream);
+}
+
+extern "C" int efm_record(const EfmDims *d, const EfmShots *sh, const EfmState *st, float *gathers,
+                          int it, void *stream) {
+    if (!d || !sh || !gathers) return fail(EFM_E_ARG, "efm_record: null argument");
+    int rc = validate_state(st);
+    if (rc) return rc;
+    if (it < 0 || it >= d->ntmax) return fail(EFM_E_ARG, "efm_record: it out of range");
+    return launch_record(make_geo(*d), make_shots(*sh), make_state(*st), gathers, it,
+                         (hipStream_t)stream);
+}
+
+extern "C" int efm_forward(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                           const EfmState *st, float *gathers, float *borders, int W, int it0,
+                           int nsteps, void *stream) {
+    int rc = validate(d, m, sh);
+    if (rc || (rc = validate_state(st))) return rc;
+    if (it0 < 0 || nsteps < 0 || it0 + nsteps > d->ntmax)
+        return fail(EFM_E_ARG, "efm_forward: step range outside [0, ntmax)");
+    if (borders && (W < 0 || W + H + 1 > d->nx / 2 || W + H + 1 > d->ny / 2 || W + H + 1 > d->nz / 2))
+        return fail(EFM_E_ARG, "efm_forward: CPML too wide for the border save");
+    const Geo g = make_geo(*d);
+    const ShotPtrs shp = make_shots(*sh);
+    const StatePtrs s = make_state(*st);
+    hipStream_t hs = (hipStream_t)stream;
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        if ((rc = launch_stress_src(g, *m, shp, s, s, it, 1, hs))) return rc;
+        if ((rc = launch_velocity(g, *m, shp, s, s, it, hs))) return rc;
+        if (gathers && (rc = launch_record(g, shp, s, gathers, it, hs))) return rc;
+        if (borders && (rc = launch_border(g, shp, s, borders, W, it, 1, it, hs))) return rc;
+    }
+    return EFM_OK;
+}
+
+extern "C" int efm_forward_states(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                                  const EfmState *const *states, float *gathers, int it0,
+                                  int nsteps, void *stream) {
+    int rc = validate(d, m, sh);
+    if (rc) return rc;
+    if (!states) return fail(EFM_E_ARG, "efm_forward_states: null state list");
+    if (it0 < 0 || nsteps < 0 || it0 + nsteps > d->ntmax)
+        return fail(EFM_E_ARG, "efm_forward_states: step range outside [0, ntmax)");
+    for (int k = 0; k <= nsteps; ++k)
+        if ((rc = validate_state(states[k]))) return rc;
+    const Geo g = make_geo(*d);
+    const ShotPtrs shp = make_shots(*sh);
+    hipStream_t hs = (hipStream_t)stream;
+    for (int k = 0; k < nsteps; ++k) {
+        const StatePtrs in = make_state(*states[k]), out = make_state(*states[k + 1]);
+        if ((rc = launch_stress_src(g, *m, shp, in, out, it0 + k, 1, hs))) return rc;
+        if ((rc = launch_velocity(g, *m, shp, in, out, it0 + k, hs))) return rc;
+        if (gathers && (rc = launch_record(g, shp, out, gathers, it0 + k, hs))) return rc;
+    }
+    return EFM_OK;
+}

@@ -1,0 +1,123 @@
+// Internal header shared by the gfx950 kernels of libefm.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "efm.h"
+
+namespace efm {
+
+constexpr int H = 2;  // interior guard: 2 <= x,y,z < n-2 (reference M:54, 77, 152)
+
+// Everything a sweep kernel needs, passed by value (kernel-argument memory).
+struct Geo {
+    int nx, ny, nz, S;
+    int wxl, wxh, wyl, wyh, wzl, wzh;  // CPML widths per face
+    int Lx, Ly, Lz;                    // slab extents wxl+wxh, ...
+    int ntmax, nrec;
+    float c;                           // signed dt/dx
+    long long N;                       // nx*ny*nz
+    long long sxz;                     // nx*nz (y stride)
+};
+
+struct StatePtrs {
+    float *v[3];    // vx vy vz
+    float *t[6];    // txx tyy tzz txy txz tyz
+    float *px, *py, *pz;
+};
+
+struct ShotPtrs {
+    const float *ax, *bx, *ay, *by, *az, *bz;
+    const int *src;
+    const float *m6;
+    const float *wav;
+    const int *nt;
+    const int *rec;
+};
+
+inline Geo make_geo(const EfmDims &d) {
+    Geo g;
+    g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.S = d.nshot;
+    g.wxl = d.wx_lo; g.wxh = d.wx_hi; g.wyl = d.wy_lo; g.wyh = d.wy_hi;
+    g.wzl = d.wz_lo; g.wzh = d.wz_hi;
+    g.Lx = d.wx_lo + d.wx_hi; g.Ly = d.wy_lo + d.wy_hi; g.Lz = d.wz_lo + d.wz_hi;
+    g.ntmax = d.ntmax; g.nrec = d.nrec; g.c = d.dtdx;
+    g.sxz = (long long)d.nx * d.nz;
+    g.N = g.sxz * d.ny;
+    return g;
+}
+
+inline StatePtrs make_state(const EfmState &s) {
+    StatePtrs p;
+    p.v[0] = s.vx; p.v[1] = s.vy; p.v[2] = s.vz;
+    p.t[0] = s.txx; p.t[1] = s.tyy; p.t[2] = s.tzz;
+    p.t[3] = s.txy; p.t[4] = s.txz; p.t[5] = s.tyz;
+    p.px = s.psi_x; p.py = s.psi_y; p.pz = s.psi_z;
+    return p;
+}
+
+inline ShotPtrs make_shots(const EfmShots &s) {
+    ShotPtrs p;
+    p.ax = s.ax; p.bx = s.bx; p.ay = s.ay; p.by = s.by; p.az = s.az; p.bz = s.bz;
+    p.src = s.src; p.m6 = s.m6; p.wav = s.wavelet; p.nt = s.nt; p.rec = s.rec;
+    return p;
+}
+
+// ---- slab addressing (ψ lives only in the CPML slabs) ----------------------
+// Returns the slab coordinate along an axis of length n with widths lo/hi, or -1.
+__device__ __forceinline__ int slab_coord(int i, int n, int lo, int hi) {
+    if (i < lo) return i;
+    if (i >= n - hi) return lo + (i - (n - hi));
+    return -1;
+}
+// psi_x (S, 6, ny, nz, Lx)
+__device__ __forceinline__ long long px_index(const Geo &g, int s, int comp, int y, int z, int sx) {
+    return ((((long long)s * 6 + comp) * g.ny + y) * g.nz + z) * g.Lx + sx;
+}
+__device__ __forceinline__ long long px_cstride(const Geo &g) { return (long long)g.ny * g.nz * g.Lx; }
+// psi_y (S, 6, Ly, nz, nx)
+__device__ __forceinline__ long long py_index(const Geo &g, int s, int comp, int sy, int z, int x) {
+    return ((((long long)s * 6 + comp) * g.Ly + sy) * g.nz + z) * g.nx + x;
+}
+__device__ __forceinline__ long long py_cstride(const Geo &g) { return (long long)g.Ly * g.nz * g.nx; }
+// psi_z (S, 6, ny, Lz, nx)
+__device__ __forceinline__ long long pz_index(const Geo &g, int s, int comp, int y, int sz, int x) {
+    return ((((long long)s * 6 + comp) * g.ny + y) * g.Lz + sz) * g.nx + x;
+}
+__device__ __forceinline__ long long pz_cstride(const Geo &g) { return (long long)g.ny * g.Lz * g.nx; }
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ bool interior(int x, int y, int z, const Geo &g) {
+    return x >= H && x < g.nx - H && y >= H && y < g.ny - H && z >= H && z < g.nz - H;
+}
+
+// ---- error plumbing ---------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int check_launch(const char *what);
+
+// ---- tile configuration chosen by the autotuner ------------------------------
+struct Tile {
+    int bx, bz, ny;  // threads in x, threads in z, cells per thread along y
+};
+Tile get_tile(int kernel);  // 0 stress, 1 velocity, 2 adjoint-velocity, 3 adjoint-stress
+int set_tile(int kernel, Tile t);
+
+// ---- launchers (defined in the .hip files) ------------------------------------
+int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                      const StatePtrs &out, int it, int src_mode, hipStream_t st);
+int launch_stress(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                  const StatePtrs &out, int it, hipStream_t st);
+int launch_velocity(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                    const StatePtrs &out, int it, hipStream_t st);
+int launch_record(const Geo &g, const ShotPtrs &sh, const StatePtrs &s, float *gathers, int it,
+                  hipStream_t st);
+int launch_border(const Geo &g, const ShotPtrs &sh, const StatePtrs &s, float *borders, int W,
+                  int slot, int save, int it_gate, hipStream_t st);
+long long border_floats(const Geo &g, int W);
+
+}  // namespace efm

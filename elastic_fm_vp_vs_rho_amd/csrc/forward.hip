@@ -1,0 +1,371 @@
+// Forward sweeps of the 3-D elastic velocity-stress staggered-grid FDTD
+// (O(2,2), CPML), written for gfx950.
+//
+// Reference: kernel_Tau (mainPropElastic3D_MT.py M:59-128), addSource
+// (M:1728-1734), kernel_Vx_Vy_Vz (M:133-188), kernel_get_Station (M:324-335),
+// kernel_get_border / kernel_set_border (M:378-462).
+//
+// Mapping: a block is BX (x, one 64-lane wave per row) × BZ (z) threads; each
+// thread owns one (x, z) column and marches NYC cells along y.  x is the
+// contiguous axis, so every wave issues fully coalesced 256-byte row loads; the
+// ±1 neighbours along x and z are the same rows shifted and are served from the
+// L1/LDS-backed vector cache of the CU, the ±y neighbour from the previous
+// march step.  ψ memory variables live only in CPML slabs (identically zero
+// elsewhere, where a = b = 0).  The moment-tensor source is added by the thread
+// that owns the source cell right after it writes τ (the reference adds it with
+// six separate gpuarray launches).
+#include "common.hpp"
+
+namespace efm {
+
+struct Med {
+    const float *lam, *mu, *rho;
+};
+
+// ---------------------------------------------------------------------------
+// stress sweep
+// ---------------------------------------------------------------------------
+template <int BX, int BZ, int NYC>
+__global__ __launch_bounds__(BX *BZ) void k_stress(Geo g, Med m, ShotPtrs sh, StatePtrs in,
+                                                  StatePtrs out, int it, int src_mode) {
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int z = blockIdx.y * BZ + threadIdx.y;
+    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
+    const int s = blockIdx.z / nyc;
+    const int yb = blockIdx.z - s * nyc;
+    if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
+    if (it >= sh.nt[s]) return;
+    const int y0 = H + yb * NYC;
+    const int y1 = min(y0 + NYC, g.ny - H);
+    const int nx = g.nx;
+    const long long sxz = g.sxz;
+    const long long so = (long long)s * g.N;
+    const float c = g.c;
+
+    const int sxs = slab_coord(x, g.nx, g.wxl, g.wxh);
+    const int szs = slab_coord(z, g.nz, g.wzl, g.wzh);
+    float ax = 0.f, bx = 0.f, az = 0.f, bz = 0.f;
+    if (sxs >= 0) { ax = sh.ax[s * g.nx + x]; bx = sh.bx[s * g.nx + x]; }
+    if (szs >= 0) { az = sh.az[s * g.nz + z]; bz = sh.bz[s * g.nz + z]; }
+    // src_mode: 1 adds M·w[it] (forward), -1 subtracts M·w[nt-1-it] (reverse-time
+    // reconstruction, flagMode = 1 in M:1964), 0 injects nothing (adjoint runs)
+    const int *sp = sh.src + 3 * s;
+    const bool src_col = (src_mode != 0) && (x == sp[0]) && (z == sp[2]);
+    const int srcy = sp[1];
+
+    const float *__restrict__ vx = in.v[0];
+    const float *__restrict__ vy = in.v[1];
+    const float *__restrict__ vz = in.v[2];
+    const float *__restrict__ mu = m.mu;
+    const float *__restrict__ lam = m.lam;
+
+    for (int y = y0; y < y1; ++y) {
+        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
+        const long long t = so + lc;
+        const float vxc = vx[t], vyc = vy[t], vzc = vz[t];
+        const float dvx_dx = vxc - vx[t - 1];
+        const float dvy_dy = vyc - vy[t - sxz];
+        const float dvz_dz = vzc - vz[t - nx];
+        const float dvx_dz = vx[t + nx] - vxc;
+        const float dvx_dy = vx[t + sxz] - vxc;
+        const float dvy_dx = vy[t + 1] - vyc;
+        const float dvy_dz = vy[t + nx] - vyc;
+        const float dvz_dx = vz[t + 1] - vzc;
+        const float dvz_dy = vz[t + sxz] - vzc;
+        // strain-like terms e = Δ + ψ(new)   (ψ = b·ψ + a·Δ, updated then used)
+        float exx = dvx_dx, eyx = dvy_dx, ezx = dvz_dx;
+        float exy = dvx_dy, eyy = dvy_dy, ezy = dvz_dy;
+        float exz = dvx_dz, eyz = dvy_dz, ezz = dvz_dz;
+        if (sxs >= 0) {
+            const long long pi = px_index(g, s, 0, y, z, sxs), cs = px_cstride(g);
+            const float p0 = bx * in.px[pi] + ax * dvx_dx;
+            const float p1 = bx * in.px[pi + cs] + ax * dvy_dx;
+            const float p2 = bx * in.px[pi + 2 * cs] + ax * dvz_dx;
+            out.px[pi] = p0; out.px[pi + cs] = p1; out.px[pi + 2 * cs] = p2;
+            exx += p0; eyx += p1; ezx += p2;
+        }
+        const int sys = slab_coord(y, g.ny, g.wyl, g.wyh);
+        if (sys >= 0) {
+            const float ay = sh.ay[s * g.ny + y], by = sh.by[s * g.ny + y];
+            const long long pi = py_index(g, s, 0, sys, z, x), cs = py_cstride(g);
+            const float p0 = by * in.py[pi] + ay * dvx_dy;
+            const float p1 = by * in.py[pi + cs] + ay * dvy_dy;
+            const float p2 = by * in.py[pi + 2 * cs] + ay * dvz_dy;
+            out.py[pi] = p0; out.py[pi + cs] = p1; out.py[pi + 2 * cs] = p2;
+            exy += p0; eyy += p1; ezy += p2;
+        }
+        if (szs >= 0) {
+            const long long pi = pz_index(g, s, 0, y, szs, x), cs = pz_cstride(g);
+            const float p0 = bz * in.pz[pi] + az * dvx_dz;
+            const float p1 = bz * in.pz[pi + cs] + az * dvy_dz;
+            const float p2 = bz * in.pz[pi + 2 * cs] + az * dvz_dz;
+            out.pz[pi] = p0; out.pz[pi + cs] = p1; out.pz[pi + 2 * cs] = p2;
+            exz += p0; eyz += p1; ezz += p2;
+        }
+        // moduli: λ, μ and the three 4-point harmonic means of μ (M:80-89)
+        const float L = lam[lc];
+        const float r0 = frcp(mu[lc]);
+        const float ri = frcp(mu[lc + 1]);
+        const float rk = frcp(mu[lc + nx]);
+        const float rj = frcp(mu[lc + sxz]);
+        const float rik = frcp(mu[lc + 1 + nx]);
+        const float rjk = frcp(mu[lc + nx + sxz]);
+        const float rij = frcp(mu[lc + 1 + sxz]);
+        const float Mu = mu[lc];
+        const float mxz = 4.f * frcp(r0 + ri + rk + rik);
+        const float mxy = 4.f * frcp(r0 + ri + rj + rij);
+        const float myz = 4.f * frcp(r0 + rj + rk + rjk);
+        const float l2m = L + 2.f * Mu;
+        float txx = in.t[0][t] + c * (l2m * exx + L * (eyy + ezz));
+        float tyy = in.t[1][t] + c * (l2m * eyy + L * (exx + ezz));
+        float tzz = in.t[2][t] + c * (l2m * ezz + L * (exx + eyy));
+        float txy = in.t[3][t] + c * mxy * (exy + eyx);
+        float txz = in.t[4][t] + c * mxz * (exz + ezx);
+        float tyz = in.t[5][t] + c * myz * (eyz + ezy);
+        if (src_col && y == srcy) {
+            const int iw = src_mode > 0 ? it : sh.nt[s] - 1 - it;
+            const float w0 = sh.wav[(long long)s * g.ntmax + iw];
+            const float w = src_mode > 0 ? w0 : -w0;
+            const float *m6 = sh.m6 + 6 * s;
+            txx += m6[0] * w; tyy += m6[1] * w; tzz += m6[2] * w;
+            txy += m6[3] * w; txz += m6[4] * w; tyz += m6[5] * w;
+        }
+        out.t[0][t] = txx; out.t[1][t] = tyy; out.t[2][t] = tzz;
+        out.t[3][t] = txy; out.t[4][t] = txz; out.t[5][t] = tyz;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// velocity sweep
+// ---------------------------------------------------------------------------
+template <int BX, int BZ, int NYC>
+__global__ __launch_bounds__(BX *BZ) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
+                                                    StatePtrs out, int it) {
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int z = blockIdx.y * BZ + threadIdx.y;
+    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
+    const int s = blockIdx.z / nyc;
+    const int yb = blockIdx.z - s * nyc;
+    if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
+    if (it >= sh.nt[s]) return;
+    const int y0 = H + yb * NYC;
+    const int y1 = min(y0 + NYC, g.ny - H);
+    const int nx = g.nx;
+    const long long sxz = g.sxz;
+    const long long so = (long long)s * g.N;
+    const float c = g.c;
+
+    const int sxs = slab_coord(x, g.nx, g.wxl, g.wxh);
+    const int szs = slab_coord(z, g.nz, g.wzl, g.wzh);
+    float ax = 0.f, bx = 0.f, az = 0.f, bz = 0.f;
+    if (sxs >= 0) { ax = sh.ax[s * g.nx + x]; bx = sh.bx[s * g.nx + x]; }
+    if (szs >= 0) { az = sh.az[s * g.nz + z]; bz = sh.bz[s * g.nz + z]; }
+
+    // τ is read from `out` (the stress sweep of this step wrote it there)
+    const float *__restrict__ txx = out.t[0];
+    const float *__restrict__ tyy = out.t[1];
+    const float *__restrict__ tzz = out.t[2];
+    const float *__restrict__ txy = out.t[3];
+    const float *__restrict__ txz = out.t[4];
+    const float *__restrict__ tyz = out.t[5];
+    const float *__restrict__ rho = m.rho;
+
+    for (int y = y0; y < y1; ++y) {
+        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
+        const long long t = so + lc;
+        const float dtxx_dx = txx[t + 1] - txx[t];
+        const float dtzz_dz = tzz[t + nx] - tzz[t];
+        const float dtyy_dy = tyy[t + sxz] - tyy[t];
+        const float txyc = txy[t], txzc = txz[t], tyzc = tyz[t];
+        const float dtxy_dy = txyc - txy[t - sxz];
+        const float dtxy_dx = txyc - txy[t - 1];
+        const float dtxz_dz = txzc - txz[t - nx];
+        const float dtxz_dx = txzc - txz[t - 1];
+        const float dtyz_dz = tyzc - tyz[t - nx];
+        const float dtyz_dy = tyzc - tyz[t - sxz];
+        float fxx = dtxx_dx, fyx = dtxy_dx, fzx = dtxz_dx;
+        float fxy = dtxy_dy, fyy = dtyy_dy, fzy = dtyz_dy;
+        float fxz = dtxz_dz, fyz = dtyz_dz, fzz = dtzz_dz;
+        if (sxs >= 0) {
+            const long long pi = px_index(g, s, 3, y, z, sxs), cs = px_cstride(g);
+            const float p0 = bx * in.px[pi] + ax * dtxx_dx;
+            const float p1 = bx * in.px[pi + cs] + ax * dtxy_dx;
+            const float p2 = bx * in.px[pi + 2 * cs] + ax * dtxz_dx;
+            out.px[pi] = p0; out.px[pi + cs] = p1; out.px[pi + 2 * cs] = p2;
+            fxx += p0; fyx += p1; fzx += p2;
+        }
+        const int sys = slab_coord(y, g.ny, g.wyl, g.wyh);
+        if (sys >= 0) {
+            const float ay = sh.ay[s * g.ny + y], by = sh.by[s * g.ny + y];
+            const long long pi = py_index(g, s, 3, sys, z, x), cs = py_cstride(g);
+            const float p0 = by * in.py[pi] + ay * dtxy_dy;
+            const float p1 = by * in.py[pi + cs] + ay * dtyy_dy;
+            const float p2 = by * in.py[pi + 2 * cs] + ay * dtyz_dy;
+            out.py[pi] = p0; out.py[pi + cs] = p1; out.py[pi + 2 * cs] = p2;
+            fxy += p0; fyy += p1; fzy += p2;
+        }
+        if (szs >= 0) {
+            const long long pi = pz_index(g, s, 3, y, szs, x), cs = pz_cstride(g);
+            const float p0 = bz * in.pz[pi] + az * dtxz_dz;
+            const float p1 = bz * in.pz[pi + cs] + az * dtyz_dz;
+            const float p2 = bz * in.pz[pi + 2 * cs] + az * dtzz_dz;
+            out.pz[pi] = p0; out.pz[pi + cs] = p1; out.pz[pi + 2 * cs] = p2;
+            fxz += p0; fyz += p1; fzz += p2;
+        }
+        // averaged buoyancy b̄ = 2bb'/(b+b') = 2/(ρ+ρ')  (M:153-159)
+        const float r = rho[lc];
+        const float bbx = 2.f * frcp(r + rho[lc + 1]);
+        const float bby = 2.f * frcp(r + rho[lc + sxz]);
+        const float bbz = 2.f * frcp(r + rho[lc + nx]);
+        out.v[0][t] = in.v[0][t] + c * bbx * (fxx + fxy + fxz);
+        out.v[1][t] = in.v[1][t] + c * bby * (fyx + fyy + fyz);
+        out.v[2][t] = in.v[2][t] + c * bbz * (fzx + fzy + fzz);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// receivers
+// ---------------------------------------------------------------------------
+__global__ void k_record(Geo g, ShotPtrs sh, StatePtrs st, float *gathers, int it) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (r >= g.nrec || it >= sh.nt[s]) return;
+    const int cell = sh.rec[(long long)s * g.nrec + r];
+    if (cell < 0) return;
+    const long long t = (long long)s * g.N + cell;
+    const long long base = ((long long)s * 3 * g.ntmax + it) * g.nrec + r;
+    const long long cstride = (long long)g.ntmax * g.nrec;
+    gathers[base] = st.v[0][t];
+    gathers[base + cstride] = st.v[1][t];
+    gathers[base + 2 * cstride] = st.v[2][t];
+}
+
+// ---------------------------------------------------------------------------
+// reference boundary save / restore of v (2 layers inside the CPML on x, y,
+// bottom; z in [2,4) at the top)
+// ---------------------------------------------------------------------------
+struct BorderLayout {
+    long long left, right, top, bottom, front, back, comp;  // offsets, comp = floats per component
+};
+__host__ __device__ inline BorderLayout border_layout(int nx, int ny, int nz) {
+    BorderLayout b;
+    const long long lr = (long long)ny * nz * H, tb = (long long)ny * H * nx, fb = (long long)H * nz * nx;
+    b.left = 0; b.right = lr; b.top = 2 * lr; b.bottom = 2 * lr + tb; b.front = 2 * lr + 2 * tb;
+    b.back = 2 * lr + 2 * tb + fb; b.comp = 2 * lr + 2 * tb + 2 * fb;
+    return b;
+}
+
+long long border_floats(const Geo &g, int W) {
+    (void)W;
+    return 3 * border_layout(g.nx, g.ny, g.nz).comp;
+}
+
+// One thread per (component, face element).  borders: (S, nslot, 3, comp)
+__global__ void k_border(Geo g, ShotPtrs sh, StatePtrs st, float *borders, int W, int slot,
+                         int nslot, int save, int it_gate) {
+    const BorderLayout L = border_layout(g.nx, g.ny, g.nz);
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (i >= 3 * L.comp) return;
+    if (it_gate >= 0 && it_gate >= sh.nt[s]) return;
+    const int comp = (int)(i / L.comp);
+    long long e = i - comp * L.comp;
+    int x, y, z;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (e < L.top) {                     // left / right: (ny, nz, H)
+        const int side = e >= L.right;
+        long long k = e - (side ? L.right : 0);
+        const int h = (int)(k % H); k /= H;
+        z = (int)(k % nz); y = (int)(k / nz);
+        x = side ? nx - W - H + h : W + h;
+    } else if (e < L.front) {            // top / bottom: (ny, H, nx)
+        const int side = e >= L.bottom;
+        long long k = e - (side ? L.bottom : L.top);
+        x = (int)(k % nx); k /= nx;
+        const int h = (int)(k % H); y = (int)(k / H);
+        z = side ? nz - W - H + h : H + h;
+    } else {                             // front / back: (H, nz, nx)
+        const int side = e >= L.back;
+        long long k = e - (side ? L.back : L.front);
+        x = (int)(k % nx); k /= nx;
+        z = (int)(k % nz); const int h = (int)(k / nz);
+        y = side ? ny - W - H + h : W + h;
+    }
+    const long long t = (long long)s * g.N + x + (long long)z * nx + (long long)y * g.sxz;
+    float *buf = borders + (((long long)s * nslot + slot) * 3 + comp) * L.comp + e;
+    if (save) *buf = st.v[comp][t];
+    else st.v[comp][t] = *buf;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define EFM_TILES(X)  \
+    X(64, 4, 1)       \
+    X(64, 4, 4)       \
+    X(64, 4, 8)       \
+    X(64, 2, 8)       \
+    X(64, 8, 4)       \
+    X(64, 4, 16)      \
+    X(128, 2, 4)      \
+    X(64, 1, 16)
+
+static inline dim3 sweep_grid(const Geo &g, const Tile &t) {
+    const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
+    return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
+}
+
+#define LAUNCH_IF(BX_, BZ_, NY_, KER, ...)                                                     \
+    if (!launched && t.bx == BX_ && t.bz == BZ_ && t.ny == NY_) {                               \
+        hipLaunchKernelGGL((KER<BX_, BZ_, NY_>), sweep_grid(g, t), dim3(BX_, BZ_), 0, st,       \
+                           __VA_ARGS__);                                                        \
+        launched = true;                                                                       \
+    }
+
+int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                      const StatePtrs &out, int it, int src_mode, hipStream_t st) {
+    const Tile t = get_tile(0);
+    Med md{m.lam, m.mu, m.rho};
+    bool launched = false;
+#define X(a, b, c) LAUNCH_IF(a, b, c, k_stress, g, md, sh, in, out, it, src_mode)
+    EFM_TILES(X)
+#undef X
+    if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported stress tile");
+    return check_launch("stress sweep");
+}
+
+int launch_stress(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                  const StatePtrs &out, int it, hipStream_t st) {
+    return launch_stress_src(g, m, sh, in, out, it, 1, st);
+}
+
+int launch_velocity(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                    const StatePtrs &out, int it, hipStream_t st) {
+    const Tile t = get_tile(1);
+    Med md{m.lam, m.mu, m.rho};
+    bool launched = false;
+#define X(a, b, c) LAUNCH_IF(a, b, c, k_velocity, g, md, sh, in, out, it)
+    EFM_TILES(X)
+#undef X
+    if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported velocity tile");
+    return check_launch("velocity sweep");
+}
+
+int launch_record(const Geo &g, const ShotPtrs &sh, const StatePtrs &s, float *gathers, int it,
+                  hipStream_t st) {
+    if (g.nrec <= 0) return EFM_OK;
+    dim3 grid((g.nrec + 63) / 64, g.S);
+    hipLaunchKernelGGL(k_record, grid, dim3(64), 0, st, g, sh, s, gathers, it);
+    return check_launch("record");
+}
+
+int launch_border(const Geo &g, const ShotPtrs &sh, const StatePtrs &s, float *borders, int W,
+                  int slot, int save, int it_gate, hipStream_t st) {
+    const long long n = 3 * border_layout(g.nx, g.ny, g.nz).comp;
+    dim3 grid((unsigned)((n + 255) / 256), g.S);
+    hipLaunchKernelGGL(k_border, grid, dim3(256), 0, st, g, sh, s, borders, W, slot, g.ntmax,
+                       save, it_gate);
+    return check_launch("border");
+}
+
+}  // namespace efm

@@ -1,0 +1,283 @@
+"""Device engine over libefm.so: wavefield storage, shot batches, forward runs
+and exact-adjoint gradients with √nt checkpointing.
+
+torch is used only to own device memory and to name the current HIP stream;
+every arithmetic operation on the path runs in the HIP kernels of libefm.so.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import EfmDims, EfmMedium, EfmShots, EfmState, check
+
+H_STENCIL = 2
+FIELDS = ("vx", "vy", "vz", "txx", "tyy", "tzz", "txy", "txz", "tyz")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@dataclass(frozen=True)
+class Grid:
+    """Grid and CPML geometry.  ``widths`` = (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi)
+    CPML cells per face; the reference uses W on all faces but z_lo (free top,
+    M:1305-1306)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dt: float
+    widths: tuple
+
+    @classmethod
+    def make(cls, nx, ny, nz, dx, dt, W, free_top=True):
+        return cls(int(nx), int(ny), int(nz), float(dx), float(dt),
+                   (W, W, W, W, 0 if free_top else W, W))
+
+    @property
+    def shape(self):
+        return (self.ny, self.nz, self.nx)
+
+    @property
+    def N(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def slab_lengths(self):
+        w = self.widths
+        return (w[0] + w[1], w[2] + w[3], w[4] + w[5])
+
+    def interior(self, x, y, z):
+        return (H_STENCIL <= x < self.nx - H_STENCIL and H_STENCIL <= y < self.ny - H_STENCIL
+                and H_STENCIL <= z < self.nz - H_STENCIL)
+
+
+class ShotBatch:
+    """Per-shot sources, receivers, wavelets and CPML profiles, on the device.
+
+    src (S, 3) integer (x, y, z); m6 (S, 6) (xx, yy, zz, xy, xz, yz); wavelet
+    (S, ntmax); nt (S,); rec (S, nrec, 3) integer (x, y, z), -1 rows unused;
+    cpml: dict of (S, n) arrays ax, bx, ay, by, az, bz."""
+
+    def __init__(self, grid: Grid, src, m6, wavelet, nt, rec, cpml, device="cuda"):
+        self.grid = grid
+        src = np.asarray(src, np.int64).reshape(-1, 3)
+        S = src.shape[0]
+        wavelet = np.asarray(wavelet, np.float32).reshape(S, -1)
+        nt = np.asarray(nt, np.int64).reshape(S)
+        rec = np.asarray(rec, np.int64).reshape(S, -1, 3)
+        for s in range(S):
+            if not grid.interior(*src[s]):
+                raise ValueError(f"source {tuple(src[s])} of shot {s} is not an interior cell "
+                                 f"(2 <= x,y,z < n-2)")
+        if np.any(nt < 1) or np.any(nt > wavelet.shape[1]):
+            raise ValueError("record lengths must lie in [1, wavelet length]")
+        used = rec[..., 0] >= 0
+        rx, ry, rz = rec[..., 0], rec[..., 1], rec[..., 2]
+        if np.any(used & ((rx >= grid.nx) | (ry < 0) | (ry >= grid.ny) | (rz < 0) | (rz >= grid.nz))):
+            raise ValueError("receiver outside the grid")
+        lin = np.where(used, rx + grid.nx * rz + grid.nx * grid.nz * ry, -1)
+        self.S, self.ntmax, self.nrec = S, wavelet.shape[1], rec.shape[1]
+        dev = torch.device(device)
+        f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.int32), device=dev)
+        self.src_host, self.nt_host, self.rec_host = src, nt, rec
+        self.src, self.m6, self.wavelet = i32(src), f32(np.asarray(m6).reshape(S, 6)), f32(wavelet)
+        self.nt, self.rec = i32(nt), i32(lin)
+        prof = {}
+        for k, n in (("ax", grid.nx), ("bx", grid.nx), ("ay", grid.ny), ("by", grid.ny),
+                     ("az", grid.nz), ("bz", grid.nz)):
+            a = np.asarray(cpml[k], np.float32)
+            prof[k] = f32(np.broadcast_to(a, (S, n)))
+        self.cpml = prof
+
+    def set_m6(self, m6):
+        self.m6.copy_(torch.as_tensor(np.asarray(m6, np.float32).reshape(self.S, 6)))
+
+    def struct(self):
+        c = self.cpml
+        return EfmShots(_ptr(c["ax"]), _ptr(c["bx"]), _ptr(c["ay"]), _ptr(c["by"]), _ptr(c["az"]),
+                        _ptr(c["bz"]), _ptr(self.src), _ptr(self.m6), _ptr(self.wavelet),
+                        _ptr(self.nt), _ptr(self.rec))
+
+
+class Wavefield:
+    """Batched state: 9 volumes (S, ny, nz, nx) in one tensor + three ψ slabs."""
+
+    def __init__(self, grid: Grid, S: int, device="cuda", psi_only_of=None):
+        self.grid, self.S = grid, S
+        Lx, Ly, Lz = grid.slab_lengths
+        ny, nz, nx = grid.shape
+        dev = torch.device(device)
+        if psi_only_of is None:
+            self.fields = torch.zeros((9, S, ny, nz, nx), dtype=torch.float32, device=dev)
+        else:
+            self.fields = psi_only_of.fields
+        self.psi_x = torch.zeros((S, 6, ny, nz, Lx), dtype=torch.float32, device=dev)
+        self.psi_y = torch.zeros((S, 6, Ly, nz, nx), dtype=torch.float32, device=dev)
+        self.psi_z = torch.zeros((S, 6, ny, Lz, nx), dtype=torch.float32, device=dev)
+        self._s = None
+
+    def __getitem__(self, name):
+        return self.fields[FIELDS.index(name)]
+
+    def struct(self):
+        if self._s is None:
+            self._s = EfmState(*[_ptr(self.fields[k]) for k in range(9)],
+                               _ptr(self.psi_x), _ptr(self.psi_y), _ptr(self.psi_z))
+        return self._s
+
+    def zero_(self):
+        self.fields.zero_(); self.psi_x.zero_(); self.psi_y.zero_(); self.psi_z.zero_()
+        return self
+
+    def copy_(self, other):
+        self.fields.copy_(other.fields); self.psi_x.copy_(other.psi_x)
+        self.psi_y.copy_(other.psi_y); self.psi_z.copy_(other.psi_z)
+        return self
+
+    def clone(self):
+        w = Wavefield.__new__(Wavefield)
+        w.grid, w.S, w._s = self.grid, self.S, None
+        w.fields, w.psi_x = self.fields.clone(), self.psi_x.clone()
+        w.psi_y, w.psi_z = self.psi_y.clone(), self.psi_z.clone()
+        return w
+
+    def nbytes(self):
+        return sum(t.numel() * 4 for t in (self.fields, self.psi_x, self.psi_y, self.psi_z))
+
+
+def lame(vp, vs, rho):
+    """λ, μ from Vp, Vs, ρ on the device (efm_model_lame)."""
+    lam = torch.empty_like(vp); mu = torch.empty_like(vp)
+    check(_capi.lib().efm_model_lame(_ptr(vp), _ptr(vs), _ptr(rho), _ptr(lam), _ptr(mu),
+                                     vp.numel(), _stream()), "efm_model_lame")
+    return lam, mu
+
+
+def lame_adjoint(vp, vs, rho, glam, gmu, grho):
+    gvp = torch.empty_like(vp); gvs = torch.empty_like(vp); gr = torch.empty_like(vp)
+    check(_capi.lib().efm_model_lame_adjoint(_ptr(vp), _ptr(vs), _ptr(rho), _ptr(glam), _ptr(gmu),
+                                             _ptr(grho), _ptr(gvp), _ptr(gvs), _ptr(gr), vp.numel(),
+                                             _stream()), "efm_model_lame_adjoint")
+    return gvp, gvs, gr
+
+
+class Propagator:
+    """Forward modelling and exact-adjoint gradients for a shot batch in one medium."""
+
+    def __init__(self, grid: Grid, lam, mu, rho, shots: ShotBatch):
+        self.grid, self.shots = grid, shots
+        for t in (lam, mu, rho):
+            if t.dtype != torch.float32 or not t.is_contiguous() or tuple(t.shape) != grid.shape:
+                raise ValueError(f"medium arrays must be contiguous float32 {grid.shape}")
+        self.lam, self.mu, self.rho = lam, mu, rho
+        self.lib = _capi.lib()
+
+    # -- descriptors --------------------------------------------------------
+    def dims(self, sign=1.0):
+        g, s = self.grid, self.shots
+        w = g.widths
+        return EfmDims(g.nx, g.ny, g.nz, s.S, w[0], w[1], w[2], w[3], w[4], w[5], s.ntmax, s.nrec,
+                       float(sign * g.dt / g.dx))
+
+    def medium(self):
+        return EfmMedium(_ptr(self.lam), _ptr(self.mu), _ptr(self.rho))
+
+    def new_wavefield(self):
+        return Wavefield(self.grid, self.shots.S, self.lam.device)
+
+    def new_gathers(self):
+        s = self.shots
+        return torch.zeros((s.S, 3, s.ntmax, s.nrec), dtype=torch.float32, device=self.lam.device)
+
+    # -- forward ------------------------------------------------------------
+    def forward(self, state=None, gathers=None, it0=0, nsteps=None, borders=None, W=0):
+        """Advance ``state`` (zeroed if None) through steps it0 .. it0+nsteps-1."""
+        st = self.new_wavefield() if state is None else state
+        g = self.new_gathers() if gathers is None else gathers
+        n = self.shots.ntmax - it0 if nsteps is None else nsteps
+        d, m, sh = self.dims(), self.medium(), self.shots.struct()
+        check(self.lib.efm_forward(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
+                                   ctypes.byref(st.struct()), _ptr(g), _ptr(borders), int(W),
+                                   int(it0), int(n), _stream()), "efm_forward")
+        return g, st
+
+    def border_buffer(self, W):
+        d = self.dims()
+        per = self.lib.efm_border_floats(ctypes.byref(d), int(W))
+        return torch.zeros((self.shots.S, self.shots.ntmax, per), dtype=torch.float32,
+                           device=self.lam.device)
+
+    # -- exact adjoint with √nt checkpointing ----------------------------------
+    def checkpoint_interval(self):
+        return max(1, int(math.ceil(math.sqrt(self.shots.ntmax))))
+
+    def forward_checkpointed(self, K=None):
+        """Forward run keeping the pre-step state every K steps."""
+        K = self.checkpoint_interval() if K is None else int(K)
+        nt = self.shots.ntmax
+        st = self.new_wavefield()
+        g = self.new_gathers()
+        snaps = []
+        for n0 in range(0, nt, K):
+            snaps.append(st.clone())
+            self.forward(st, g, it0=n0, nsteps=min(K, nt - n0))
+        return g, snaps, K
+
+    def gradient_from_snapshots(self, snaps, K, resid):
+        """∂J/∂λ, ∂J/∂μ, ∂J/∂ρ (volumes) and ∂J/∂M (S, 6) float64 for
+        ∂J/∂gathers = resid (S, 3, ntmax, nrec)."""
+        s, g = self.shots, self.grid
+        dev = self.lam.device
+        resid = resid.to(device=dev, dtype=torch.float32).contiguous()
+        if tuple(resid.shape) != (s.S, 3, s.ntmax, s.nrec):
+            raise ValueError(f"residual must be {(s.S, 3, s.ntmax, s.nrec)}")
+        d, m, sh = self.dims(), self.medium(), s.struct()
+        mbar = torch.empty((3,) + g.shape, dtype=torch.float32, device=dev)
+        bbar = torch.empty_like(mbar)
+        check(self.lib.efm_derived_medium(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
+                                          _stream()), "efm_derived_medium")
+        nt = s.ntmax
+        bufs = [self.new_wavefield() for _ in range(min(K, nt) + 1)]
+        adj = self.new_wavefield()
+        adj2 = Wavefield(g, s.S, dev, psi_only_of=adj)
+        acc = torch.zeros((s.S, 8) + g.shape, dtype=torch.float32, device=dev)
+        gm6 = torch.zeros((s.S, 6), dtype=torch.float64, device=dev)
+        for seg in range(len(snaps) - 1, -1, -1):
+            n0 = seg * K
+            n1 = min(n0 + K, nt)
+            k = n1 - n0
+            bufs[0].copy_(snaps[seg])
+            arr = (ctypes.POINTER(EfmState) * (k + 1))(*[ctypes.pointer(b.struct()) for b in bufs[:k + 1]])
+            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), arr,
+                                              None, n0, k, _stream()), "efm_forward_states")
+            check(self.lib.efm_adjoint(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
+                                       ctypes.byref(sh), arr, ctypes.byref(adj.struct()),
+                                       ctypes.byref(adj2.struct()), _ptr(resid), _ptr(acc), _ptr(gm6),
+                                       n0, n1, int(n0 == 0), _stream()), "efm_adjoint")
+        glam = torch.empty(g.shape, dtype=torch.float32, device=dev)
+        gmu = torch.empty_like(glam)
+        grho = torch.empty_like(glam)
+        check(self.lib.efm_gradient_combine(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
+                                            _ptr(acc), _ptr(glam), _ptr(gmu), _ptr(grho), _stream()),
+              "efm_gradient_combine")
+        return glam, gmu, grho, gm6
+
+    def gradient(self, obs, K=None):
+        """Misfit J = ½‖d - obs‖² and its exact gradient (the FWI gradient)."""
+        g, snaps, K = self.forward_checkpointed(K)
+        resid = g - obs
+        J = 0.5 * float(torch.sum(resid.double() ** 2))
+        return (J, g) + self.gradient_from_snapshots(snaps, K, resid)

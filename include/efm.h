@@ -1,0 +1,169 @@
+/*
+ * efm.h — C ABI of libefm.so, the gfx950 (MI355X) elastic FDTD propagator.
+ *
+ * The library replaces the device side of mainPropElastic3D_MT.py (the PyCUDA
+ * SourceModule kernels, M:42-530, and the per-step launch sequences of
+ * Elastic3D, M:1491-2082).  Every entry point is asynchronous on the HIP
+ * stream it is given, allocates nothing, and returns 0 or a negative EFM_E_*
+ * code; efm_last_error() returns the message of the last failure on the
+ * calling thread.
+ *
+ * Memory conventions (all pointers are device pointers, float32 unless noted):
+ *   volume           (ny, nz, nx), x fastest — the reference layout (M:72, 758)
+ *   batched volume   (S, ny, nz, nx), shot-major
+ *   ψ slabs          psi_x (S, 6, ny, nz, Lx), psi_y (S, 6, Ly, nz, nx),
+ *                    psi_z (S, 6, ny, Lz, nx) with L = lo + hi CPML widths of
+ *                    the axis.  Components: x: vx_x vy_x vz_x txx_x txy_x txz_x;
+ *                    y: vx_y vy_y vz_y txy_y tyy_y tyz_y; z: vx_z vy_z vz_z
+ *                    txz_z tyz_z tzz_z.  (The reference keeps 18 full volumes,
+ *                    M:811-862; ψ is identically zero outside the CPML.)
+ *   gathers          (S, 3, ntmax, nrec): vx, vy, vz, time-major per shot like
+ *                    the reference Gather (M:1132-1140)
+ *   receivers        (S, nrec) int32 linear cell index x + nx*z + nx*nz*y,
+ *                    -1 = unused slot
+ *   sources          (S, 3) int32 (x, y, z); must be interior (2 <= . < n-2)
+ *   moment tensors   (S, 6) float32 (xx, yy, zz, xy, xz, yz) (M:1374-1389)
+ */
+#ifndef EFM_H
+#define EFM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFM_OK 0
+#define EFM_E_ARG (-1)
+#define EFM_E_HIP (-2)
+#define EFM_E_UNSUPPORTED (-3)
+
+typedef struct EfmDims {
+    int nx, ny, nz;   /* grid */
+    int nshot;        /* S */
+    int wx_lo, wx_hi, wy_lo, wy_hi, wz_lo, wz_hi;  /* CPML widths per face (ψ slab extents) */
+    int ntmax;        /* time samples per shot buffer (gathers, wavelets) */
+    int nrec;         /* receiver slots per shot */
+    float dtdx;       /* signed dt/dx; the reference's flagMode sign (M:1502) */
+} EfmDims;
+
+typedef struct EfmMedium {     /* (ny, nz, nx), shared by all shots */
+    const float *lam, *mu, *rho;
+} EfmMedium;
+
+typedef struct EfmState {      /* batched wavefield (S, ny, nz, nx) + ψ slabs */
+    float *vx, *vy, *vz, *txx, *tyy, *tzz, *txy, *txz, *tyz;
+    float *psi_x, *psi_y, *psi_z;
+} EfmState;
+
+typedef struct EfmShots {      /* per-shot description */
+    const float *ax, *bx;      /* (S, nx) CPML profiles (CPML.CPML_i, M:1340-1372) */
+    const float *ay, *by;      /* (S, ny) */
+    const float *az, *bz;      /* (S, nz) */
+    const int   *src;          /* (S, 3) */
+    const float *m6;           /* (S, 6) */
+    const float *wavelet;      /* (S, ntmax) */
+    const int   *nt;           /* (S) record length of each shot */
+    const int   *rec;          /* (S, nrec) */
+} EfmShots;
+
+/* ---- library ---------------------------------------------------------- */
+int efm_version(void);
+/* Copies the last error message of the calling thread into buf (NUL-terminated). */
+int efm_last_error(char *buf, int len);
+
+/* ---- tile selection (no reference counterpart: the reference hard-codes
+ * block=(32,2,4), M:2764-2766).  kernel: 0 stress, 1 velocity, 2 adjoint
+ * velocity, 3 adjoint stress.  A tile is bx × bz threads, ny cells per thread
+ * along y.  efm_tile_variants writes the compiled tiles as triples and returns
+ * their count.  The autotuner (autotune.py) drives these. */
+int efm_set_tile(int kernel, int bx, int bz, int ny);
+int efm_get_tile(int kernel, int *bx, int *bz, int *ny);
+int efm_tile_variants(int *out, int max);
+
+/* ---- model maps -------------------------------------------------------- */
+/* Model.compute_Lambda_Mu (M:1196-1198): mu = vs*vs*rho, lam = vp*vp*rho - 2*mu,
+ * float32 with no contraction (bit-identical to the reference's numpy). */
+int efm_model_lame(const float *vp, const float *vs, const float *rho,
+                   float *lam, float *mu, long long n, void *stream);
+/* Chain rule back to velocities: gvp = 2ρVp·gλ, gvs = 2ρVs·(gμ - 2gλ),
+ * grho_out = (Vp²-2Vs²)·gλ + Vs²·gμ + grho_in. */
+int efm_model_lame_adjoint(const float *vp, const float *vs, const float *rho,
+                           const float *glam, const float *gmu, const float *grho_in,
+                           float *gvp, float *gvs, float *grho_out, long long n, void *stream);
+
+/* ---- forward sweeps ---------------------------------------------------- */
+/* kernel_Tau (M:59-128) + addSource (M:1728-1734): τ/ψ_v of `out` from v, τ,
+ * ψ_v of `in` (in may alias out); the source term M·w[it] is added to all six
+ * τ at each shot's source cell.  Shots with it >= nt[s] are left untouched. */
+int efm_stress_step(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                    const EfmState *in, const EfmState *out, int it, void *stream);
+/* kernel_Vx_Vy_Vz (M:133-188): v/ψ_τ of `out` from τ of `out` and v, ψ_τ of `in`. */
+int efm_velocity_step(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                      const EfmState *in, const EfmState *out, int it, void *stream);
+/* kernel_get_Station (M:324-335): gathers[s, c, it, r] = v_c[rec[s, r]]. */
+int efm_record(const EfmDims *d, const EfmShots *sh, const EfmState *st,
+               float *gathers, int it, void *stream);
+/* forwardFields' loop (M:1812-1848) for steps it0 .. it0+nsteps-1, in place:
+ * stress+source, velocity, record.  `borders` (nullable) receives the
+ * reference boundary save of v every step (getBorder_Vx_Vy_Vz, M:1524-1562);
+ * W is the CPML width used for it. */
+int efm_forward(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                const EfmState *st, float *gathers, float *borders, int W,
+                int it0, int nsteps, void *stream);
+/* Out-of-place forward for steps it0 .. it0+nsteps-1 (no reference
+ * counterpart; used to rebuild a checkpoint segment): states[k] holds the
+ * pre-step state S^(it0+k), k = 0 .. nsteps; step it0+k reads states[k] and
+ * writes states[k+1].  Only interior cells are written, so the state buffers
+ * must start zeroed outside the interior.  gathers is nullable. */
+int efm_forward_states(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                       const EfmState *const *states, float *gathers, int it0, int nsteps,
+                       void *stream);
+
+/* ---- exact discrete adjoint -------------------------------------------- */
+/* Derived medium for the adjoint: mbar (3, ny, nz, nx) = harmonic-mean μ_xy,
+ * μ_xz, μ_yz; bbar (3, ...) = 2/(ρ+ρ') along x, y, z.  Interior only. */
+int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float *bbar,
+                       void *stream);
+/* Reverse sweep of steps n = n1-1 down to n0 (n0 < n1).  fwd[k] is the
+ * forward pre-step state S^(n0+k), k = 0 .. n1-n0 (so fwd[n1-n0] is S^n1).
+ * adj/adj2 are the adjoint state and a second ψ buffer (ping-pong; on return
+ * the live ψ adjoint is in adj).  resid is ∂J/∂gathers (S, 3, ntmax, nrec).
+ * acc (S, 8, ny, nz, nx) accumulates λ, μ_n, μ_xy, μ_xz, μ_yz, ρx, ρy, ρz
+ * sums; gm6 (S, 6) double accumulates ∂J/∂M.  `last` = 1 when n0 is the
+ * first step of the run (closes the moment-tensor sum of step n0). */
+int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mbar, const float *bbar,
+                const EfmShots *sh, const EfmState *const *fwd, const EfmState *adj,
+                const EfmState *adj2, const float *resid, float *acc, double *gm6,
+                int n0, int n1, int last, void *stream);
+/* ∂J/∂λ, ∂J/∂μ, ∂J/∂ρ (ny, nz, nx) summed over shots from acc. */
+int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const float *mbar,
+                         const float *bbar, const float *acc, float *glam, float *gmu,
+                         float *grho, void *stream);
+
+/* ---- reference-style adjoint and imaging (drop-in parity) -------------- */
+/* kernel_add_adjoint_source (M:310-322): v[rec] += res[nt-it-1]. */
+int efm_inject_residual(const EfmDims *d, const EfmShots *sh, const EfmState *st,
+                        const float *res, int it, void *stream);
+/* kernel_Gradients_Mij (M:337-372), accumulated: gm6[s] += w[nt-it-1]·C(τ†(src)). */
+int efm_image_mt_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                     const EfmState *adj, double *gm6, int it, void *stream);
+/* backwardFields (M:2056-2082, mode 0): nt adjoint steps with the forward
+ * operator (+dt), residual injection and MT imaging; adj must be zero. */
+int efm_backward_mt_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                        const EfmState *adj, const float *res, double *gm6, void *stream);
+/* backwardFields_FowardFieldsRemake (M:1929-2054, mode 1), nt-1 iterations.
+ * fwd is the final forward state (consumed), vpres is a 3-volume scratch
+ * (S, 3, ny, nz, nx), borders from efm_forward, adj zero on entry.  Imaging
+ * kernel_Gradients_Rho/_Lambda/_Mu (M:191-294) accumulated into glam, gmu,
+ * grho (S, ny, nz, nx) in float32 like the reference's host sums. */
+int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                            const EfmState *fwd, float *vpres, const float *borders, int W,
+                            const EfmState *adj, const float *res, float dt, float dx,
+                            float dy, float dz, float *glam, float *gmu, float *grho,
+                            void *stream);
+/* Size in floats of the boundary buffer efm_forward writes for one shot. */
+long long efm_border_floats(const EfmDims *d, int W);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

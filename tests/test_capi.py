@@ -1,0 +1,76 @@
+"""The C ABI library loads, exports every symbol include/efm.h declares, and
+rejects bad arguments before touching a GPU (CPU-only test)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "efm.h")) as f:
+        src = f.read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(efm_[a-z_0-9]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from elastic_fm_vp_vs_rho_amd import _capi
+    return _capi.lib()
+
+
+def test_every_declared_symbol_is_exported_and_bound(L):
+    from elastic_fm_vp_vs_rho_amd import _capi
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_capi.exported_symbols())
+
+
+def test_single_hip_runtime_mapped(L):
+    from elastic_fm_vp_vs_rho_amd import _capi
+    rts = _capi._hip_runtimes_mapped()
+    assert len(rts) == 1 and "torch" in rts[0]
+
+
+def test_version_and_errors(L):
+    from elastic_fm_vp_vs_rho_amd import _capi
+    assert L.efm_version() == 10000
+    rc = L.efm_forward(None, None, None, None, None, None, 0, 0, 1, None)
+    assert rc == -1
+    assert "null" in _capi.last_error()
+    assert L.efm_set_tile(0, 48, 3, 5) == -3
+    assert "tile" in _capi.last_error()
+    n = L.efm_tile_variants(None, 0)
+    buf = (ctypes.c_int * (3 * n))()
+    assert L.efm_tile_variants(buf, n) == n
+    assert all(buf[3 * i] % 64 == 0 for i in range(n))   # whole 64-lane waves along x
+
+
+def test_argument_validation_without_gpu(L):
+    from elastic_fm_vp_vs_rho_amd._capi import EfmDims, EfmMedium, EfmShots, EfmState
+    d = EfmDims(4, 10, 10, 1, 0, 0, 0, 0, 0, 0, 10, 0, 0.1)   # nx < 5
+    m = EfmMedium(1, 1, 1)
+    sh = EfmShots(*([1] * 11))
+    st = EfmState(*([1] * 12))
+    assert L.efm_forward(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(st),
+                         None, None, 0, 0, 1, None) == -1
+    d = EfmDims(10, 10, 10, 1, 3, 3, 3, 3, 0, 3, 10, 0, 0.1)
+    assert L.efm_forward(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(st),
+                         None, None, 0, 5, 6, None) == -1   # step range past ntmax
+
+
+def test_shot_batch_validation():
+    """Host-side validation of sources/receivers happens before any upload."""
+    import numpy as np
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, ShotBatch
+    g = Grid.make(16, 16, 16, 10.0, 1e-3, 3)
+    cp = {k: np.zeros(16, np.float32) for k in ("ax", "bx", "ay", "by", "az", "bz")}
+    with pytest.raises(ValueError, match="interior"):
+        ShotBatch(g, [(1, 8, 8)], [np.zeros(6)], [np.zeros(10)], [10], [[(5, 5, 2)]], cp, device="cpu")
+    with pytest.raises(ValueError, match="receiver"):
+        ShotBatch(g, [(8, 8, 8)], [np.zeros(6)], [np.zeros(10)], [10], [[(5, 16, 2)]], cp, device="cpu")

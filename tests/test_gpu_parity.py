@@ -1,0 +1,129 @@
+"""GPU parity of the HIP propagator against the FP64 oracle (called through the C ABI).
+
+Tolerances (relative L2, FP32 device vs FP64 oracle on identical float32 inputs):
+one step 1e-6 per field; seismograms 1e-5; exact gradients 1e-4 per field.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import elastic_fp64 as O
+from tests.gpu_helpers import propagator_of, random_state, state_from_device, state_to_device
+from tests.problems import rel_l2, run_forward, tiny
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def _step_gpu(prop, wf, it):
+    from elastic_fm_vp_vs_rho_amd import _capi
+    from elastic_fm_vp_vs_rho_amd.propagator import _stream
+    L = _capi.lib()
+    d, m, sh = prop.dims(), prop.medium(), prop.shots.struct()
+    s = wf.struct()
+    _capi.check(L.efm_stress_step(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(s),
+                                  ctypes.byref(s), it, _stream()), "stress")
+    _capi.check(L.efm_velocity_step(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(s),
+                                    ctypes.byref(s), it, _stream()), "velocity")
+
+
+@pytest.mark.parametrize("free_top", [False, True])
+def test_one_step_from_random_state(gpu_lib, free_top):
+    p = tiny(n=(70, 13, 12), W=3, seed=3, free_top=free_top)
+    rng = np.random.default_rng(8)
+    F = random_state(p, rng)
+    prop = propagator_of(p)
+    wf = prop.new_wavefield()
+    state_to_device(F, wf)
+    _step_gpu(prop, wf, 5)
+    torch.cuda.synchronize()
+    G = state_from_device(wf, F.shape)
+    med, cp = p.medium(), p.cpml()
+    O.stress_sweep(F, med, cp, p.c)
+    O.add_source(F, p.src, p.m6, float(p.wavelet[5]))
+    O.velocity_sweep(F, med, cp, p.c)
+    for k in O.FIELDS:
+        assert rel_l2(G.f[k], F.f[k]) < 1e-6, k
+    for k in O.PSI_V + O.PSI_T:
+        if np.any(F.psi[k]):
+            assert rel_l2(G.psi[k], F.psi[k]) < 1e-6, k
+
+
+@pytest.mark.parametrize("free_top", [False, True])
+def test_forward_seismograms_tiny(gpu_lib, free_top):
+    p = tiny(n=(40, 22, 18), nt=150, W=5, seed=4, free_top=free_top, nrec=10)
+    ref, _ = run_forward(p)
+    prop = propagator_of(p)
+    g, _ = prop.forward()
+    got = g[0].double().cpu().numpy()
+    assert np.abs(ref).max() > 0
+    assert rel_l2(got, ref) < SEIS_TOL
+
+
+def test_forward_is_deterministic_and_batch_equals_single(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd.propagator import Propagator, ShotBatch
+    from tests.gpu_helpers import grid_of
+    p = tiny(n=(36, 20, 16), nt=80, W=4, seed=6, nrec=7)
+    grid = grid_of(p)
+    rng = np.random.default_rng(0)
+    srcs = [(10, 9, 8), (20, 12, 7), (25, 6, 10)]
+    m6s = rng.uniform(-1, 1, (3, 6))
+    nts = [80, 55, 71]
+    W = np.stack([p.wavelet] * 3)
+    dev = torch.device("cuda")
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+    lam, mu, rho = f(p.lam), f(p.mu), f(p.rho)
+    batch = Propagator(grid, lam, mu, rho, ShotBatch(grid, srcs, m6s, W, nts, [p.rec] * 3, p.cp))
+    gb, _ = batch.forward()
+    for s in range(3):
+        single = Propagator(grid, lam, mu, rho, ShotBatch(grid, [srcs[s]], [m6s[s]], W[s:s + 1], [nts[s]],
+                                                          [p.rec], p.cp))
+        g1, _ = single.forward()
+        assert torch.equal(gb[s, :, :nts[s]], g1[0, :, :nts[s]]), s
+        assert not torch.any(gb[s, :, nts[s]:])
+
+
+def _gradient_problem():
+    return tiny(n=(30, 20, 18), nt=120, W=4, seed=12, nrec=9)
+
+
+def test_exact_gradient_matches_oracle(gpu_lib):
+    p = _gradient_problem()
+    obs = run_forward(p, lam=p.lam * 1.02, rho=p.rho * 0.99)[0].astype(np.float32)
+    hist = []
+    g, _ = run_forward(p, history=hist)
+    r, _ = O.misfit(g, obs)
+    ref = O.exact_adjoint(p.medium(), p.cpml(), p.c, p.src, p.wavelet, p.rec, p.nt, r, hist)
+    prop = propagator_of(p)
+    J, gg, glam, gmu, grho, gm6 = prop.gradient(torch.as_tensor(obs[None], device="cuda"), K=11)
+    assert rel_l2(gg[0].double().cpu().numpy(), g) < SEIS_TOL
+    for name, got in (("lam", glam), ("mu", gmu), ("rho", grho)):
+        e = rel_l2(got.double().cpu().numpy(), ref[name])
+        assert e < GRAD_TOL, (name, e)
+    assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < GRAD_TOL
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(gpu_lib):
+    p = _gradient_problem()
+    prop = propagator_of(p)
+    obs = torch.zeros((1, 3, p.nt, len(p.rec)), device="cuda")
+    outs = [prop.gradient(obs, K=K) for K in (1, 7, p.nt)]
+    for o in outs[1:]:
+        for a, b in zip(outs[0][2:], o[2:]):
+            assert torch.equal(a, b)
+
+
+def test_gpu_dot_product(gpu_lib):
+    """⟨F m, r⟩ = ⟨m, Fᵀ r⟩ for the moment-tensor map, in FP32 on the device."""
+    p = tiny(n=(34, 26, 22), nt=160, W=5, seed=15, nrec=12)
+    prop = propagator_of(p)
+    g, snaps, K = prop.forward_checkpointed()
+    r = torch.randn(g.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+    _, _, _, gm6 = prop.gradient_from_snapshots(snaps, K, r)
+    lhs = float(torch.sum(g.double() * r.double()))
+    rhs = float(np.dot(p.m6, gm6[0].cpu().numpy()))
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
