@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark: exact FWI gradient (forward + adjoint) of the 128³ three-layer
+model with a Gaussian anomaly, 2000 steps, 64 receivers, one MT source, on one
+MI355X.  A bench step is one full gradient evaluation: forward run with √nt
+checkpoints, misfit, reverse sweep (segment re-forward + adjoint + imaging),
+gradient assembly.
+
+Prints one JSON line.  ``value`` is gradient cell-steps per second (grid cells
+× time steps of one gradient, per second of wall time); one gradient cell-step
+is one cell carried through one forward and one adjoint time step.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt updates/s + s per FWI gradient, batched vs plain, at ≤1e-5 err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per interior cell for one launch of each kernel (each field
+# read once and written once; neighbours reused on chip; ψ, source, receiver
+# traffic excluded):
+#   stress     v×3, λ, μ read; τ×6 read+write                    = 17 words
+#   velocity   τ×6, ρ read; v×3 read+write                        = 13 words
+#   adj_vel    v̄×3, forward τ×6, b̄×3 read; τ̄×6 and ρ-sums×3 r+w   = 30 words
+#   adj_str    τ̄×6, forward v×3, λ, μ, μ̄×3 read; v̄×3, λμ-sums×5 r+w = 30 words
+ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 120}
+
+
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def time_kernels(prop, reps=30):
+    """Average duration (ms) of each sweep kernel, measured with HIP events on
+    the stream the library launches on (torch's current stream)."""
+    import torch
+    from elastic_fm_vp_vs_rho_amd import _capi
+    from elastic_fm_vp_vs_rho_amd.propagator import _ptr, _stream
+    L = _capi.lib()
+    d, m, sh = prop.dims(), prop.medium(), prop.shots.struct()
+    g, st = prop.forward(nsteps=200)                       # a live wavefield
+    s = st.struct()
+    out = {}
+
+    def timed(fn):
+        fn()
+        a, b = _events()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    out["stress"] = timed(lambda: _capi.check(L.efm_stress_step(
+        ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(s), ctypes.byref(s), 100,
+        _stream()), "stress"))
+    out["velocity"] = timed(lambda: _capi.check(L.efm_velocity_step(
+        ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(s), ctypes.byref(s), 100,
+        _stream()), "velocity"))
+    # one adjoint step = points kernel + adjoint velocity + adjoint stress sweeps
+    from elastic_fm_vp_vs_rho_amd.propagator import Wavefield
+    nxt = st.clone()
+    mbar = torch.zeros((3,) + prop.grid.shape, device=prop.lam.device)
+    bbar = torch.zeros_like(mbar)
+    _capi.check(L.efm_derived_medium(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar), _stream()))
+    adj = st.clone()
+    adj2 = Wavefield(prop.grid, prop.shots.S, prop.lam.device, psi_only_of=adj)
+    acc = torch.zeros((prop.shots.S, 8) + prop.grid.shape, device=prop.lam.device)
+    gm6 = torch.zeros((prop.shots.S, 6), dtype=torch.float64, device=prop.lam.device)
+    resid = torch.zeros_like(g)
+    arr = (ctypes.POINTER(_capi.EfmState) * 2)(ctypes.pointer(st.struct()), ctypes.pointer(nxt.struct()))
+    out["adjoint_step"] = timed(lambda: _capi.check(L.efm_adjoint(
+        ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar), ctypes.byref(sh), arr,
+        ctypes.byref(adj.struct()), ctypes.byref(adj2.struct()), _ptr(resid), _ptr(acc), _ptr(gm6),
+        100, 101, 0, _stream()), "adjoint"))
+    return out
+
+
+def cpu_baseline(wl, steps):
+    """FP64 NumPy oracle: the same gradient (forward with stored history + exact
+    adjoint) on the first ``steps`` time steps of the workload, one thread."""
+    import numpy as np
+    from oracle import elastic_fp64 as O
+    lam, mu = wl.lame()
+    med = O.Medium(lam, mu, wl.rho)
+    cp = O.Cpml.from_dict({k: v[0] for k, v in wl.cpml().items()})
+    w = wl.wavelets()[0]
+    rec = wl.rec[0]
+    c = wl.dt / wl.dx
+    t0 = time.perf_counter()
+    hist = []
+    g, _ = O.forward(med, cp, c, tuple(wl.src[0]), wl.m6[0], w, rec, steps, threads=1, history=hist)
+    O.exact_adjoint(med, cp, c, tuple(wl.src[0]), w, rec, steps, g, hist)
+    dt = time.perf_counter() - t0
+    return dict(value=wl.nx * wl.ny * wl.nz * steps / dt, unit="gradient cell-steps/s", cores=1,
+                kind="port", sample=f"first {steps} of {wl.ntmax} time steps of the same gradient "
+                f"(forward with stored history + exact adjoint), {wl.nx}x{wl.ny}x{wl.nz}, FP64 NumPy, "
+                f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="grid size (default 128: the C2 workload)")
+    ap.add_argument("--nt", type=int, default=2000)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kernels", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every GPU box of this build has one MI355X and the path "
+              "shards by shot with no cross-GPU work; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from elastic_fm_vp_vs_rho_amd import workloads
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch, lame
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    wl = workloads.c2(n=args.n, nt=args.nt, anomaly=False)
+    wt = workloads.c2(n=args.n, nt=args.nt, anomaly=True)
+    grid = Grid(wl.nx, wl.ny, wl.nz, wl.dx, wl.dt, (wl.W,) * 6)
+    shots = ShotBatch(grid, wl.src, wl.m6, wl.wavelets(), wl.nt, wl.rec, wl.cpml(), device=dev)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    lam_t, mu_t = lame(f(wt.vp), f(wt.vs), f(wt.rho))
+    true = Propagator(grid, lam_t, mu_t, f(wt.rho), shots)
+    obs, _ = true.forward()
+    lam, mu = lame(f(wl.vp), f(wl.vs), f(wl.rho))
+    prop = Propagator(grid, lam, mu, f(wl.rho), shots)
+    K = prop.checkpoint_interval()
+
+    for _ in range(args.warmup):
+        prop.gradient(obs, K)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        J, g, glam, gmu, grho, gm6 = prop.gradient(obs, K)
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    s_per_grad = T / max(args.steps, 1)
+    N = grid.N
+    nt = wl.ntmax
+    value = N * nt * wl.S / s_per_grad
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "gradient cell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * s_per_grad,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "grid": [wl.nx, wl.ny, wl.nz], "nt": nt, "shots": wl.S,
+                   "receivers": int(wl.rec.shape[1]), "checkpoint_interval": K,
+                   "gradient": "exact discrete adjoint, sqrt(nt) checkpoints"},
+        "s_per_gradient": s_per_grad,
+        "forward_grid_pt_updates_per_s": None,
+        "misfit": J,
+    }
+    # forward-only throughput of the same model (plain modelling path)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prop.forward()
+    torch.cuda.synchronize()
+    line["forward_grid_pt_updates_per_s"] = N * nt / (time.perf_counter() - t1)
+
+    if not args.no_kernels:
+        kt = time_kernels(prop)
+        interior = (wl.nx - 4) * (wl.ny - 4) * (wl.nz - 4)
+        per = {k: ALG_BYTES[k] * interior / (kt[k] * 1e-3) / 1e9 for k in ("stress", "velocity")}
+        adj_gbs = (ALG_BYTES["adj_velocity"] + ALG_BYTES["adj_stress"]) * interior / (kt["adjoint_step"] * 1e-3) / 1e9
+        dom = max(kt, key=lambda k: kt[k])
+        if dom == "adjoint_step":
+            achieved, name, alg = adj_gbs, "adjoint step (adj_velocity + adj_stress + points)", 240
+        else:
+            achieved, name, alg = per[dom], dom, ALG_BYTES[dom]
+        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+                            "alg_bytes_per_cell": alg, "cells_per_launch": interior,
+                            "avg_ms": kt[dom], "all_kernels_ms": kt,
+                            "all_kernels_gbs": dict(per, adjoint_step=adj_gbs)}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -107,6 +107,25 @@ def test_exact_gradient_matches_oracle(gpu_lib):
     assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < GRAD_TOL
 
 
+def test_exact_adjoint_given_the_oracle_residual(gpu_lib):
+    """Adjoint precision alone: the device adjoint is driven by the oracle's
+    residual, so forward FP32 error cannot leak in through the misfit."""
+    p = _gradient_problem()
+    obs = run_forward(p, mu=p.mu * 1.03)[0]
+    hist = []
+    g, _ = run_forward(p, history=hist)
+    r, _ = O.misfit(g, obs)
+    ref = O.exact_adjoint(p.medium(), p.cpml(), p.c, p.src, p.wavelet, p.rec, p.nt, r, hist)
+    prop = propagator_of(p)
+    _, snaps, K = prop.forward_checkpointed(K=13)
+    glam, gmu, grho, gm6 = prop.gradient_from_snapshots(
+        snaps, K, torch.as_tensor(r[None].astype(np.float32), device="cuda"))
+    for name, got in (("lam", glam), ("mu", gmu), ("rho", grho)):
+        e = rel_l2(got.double().cpu().numpy(), ref[name])
+        assert e < 1e-5, (name, e)
+    assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < 1e-5
+
+
 def test_checkpoint_interval_does_not_change_the_gradient(gpu_lib):
     p = _gradient_problem()
     prop = propagator_of(p)
