@@ -40,50 +40,13 @@ def _events():
 
 
 def time_kernels(prop, reps=30):
-    """Average duration (ms) of each sweep kernel, measured with HIP events on
-    the stream the library launches on (torch's current stream)."""
-    import torch
-    from elastic_fm_vp_vs_rho_amd import _capi
-    from elastic_fm_vp_vs_rho_amd.propagator import _ptr, _stream
-    L = _capi.lib()
-    d, m, sh = prop.dims(), prop.medium(), prop.shots.struct()
-    g, st = prop.forward(nsteps=200)                       # a live wavefield
-    s = st.struct()
-    out = {}
-
-    def timed(fn):
-        fn()
-        a, b = _events()
-        a.record()
-        for _ in range(reps):
-            fn()
-        b.record()
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps
-
-    out["stress"] = timed(lambda: _capi.check(L.efm_stress_step(
-        ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(s), ctypes.byref(s), 100,
-        _stream()), "stress"))
-    out["velocity"] = timed(lambda: _capi.check(L.efm_velocity_step(
-        ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(s), ctypes.byref(s), 100,
-        _stream()), "velocity"))
-    # one adjoint step = points kernel + adjoint velocity + adjoint stress sweeps
-    from elastic_fm_vp_vs_rho_amd.propagator import Wavefield
-    nxt = st.clone()
-    mbar = torch.zeros((3,) + prop.grid.shape, device=prop.lam.device)
-    bbar = torch.zeros_like(mbar)
-    _capi.check(L.efm_derived_medium(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar), _stream()))
-    adj = st.clone()
-    adj2 = Wavefield(prop.grid, prop.shots.S, prop.lam.device, psi_only_of=adj)
-    acc = torch.zeros((prop.shots.S, 8) + prop.grid.shape, device=prop.lam.device)
-    gm6 = torch.zeros((prop.shots.S, 6), dtype=torch.float64, device=prop.lam.device)
-    resid = torch.zeros_like(g)
-    arr = (ctypes.POINTER(_capi.EfmState) * 2)(ctypes.pointer(st.struct()), ctypes.pointer(nxt.struct()))
-    out["adjoint_step"] = timed(lambda: _capi.check(L.efm_adjoint(
-        ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar), ctypes.byref(sh), arr,
-        ctypes.byref(adj.struct()), ctypes.byref(adj2.struct()), _ptr(resid), _ptr(acc), _ptr(gm6),
-        100, 101, 0, _stream()), "adjoint"))
-    return out
+    """Average duration (ms) of each sweep kernel with the installed tiles,
+    measured with HIP events on the stream the library launches on (torch's
+    current stream).  The adjoint pair is timed together (one adjoint step =
+    points kernel + adjoint velocity + adjoint stress sweeps)."""
+    from elastic_fm_vp_vs_rho_amd.autotune import _Bench
+    B = _Bench(prop, warm_steps=100)
+    return {"stress": B.time(0, reps), "velocity": B.time(1, reps), "adjoint_step": B.time(2, reps)}
 
 
 def cpu_baseline(wl, steps):
@@ -118,6 +81,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every GPU box of this build has one MI355X and the path "
@@ -142,6 +106,12 @@ def main():
     lam, mu = lame(f(wl.vp), f(wl.vs), f(wl.rho))
     prop = Propagator(grid, lam, mu, f(wl.rho), shots)
     K = prop.checkpoint_interval()
+    tiles = None
+    if not args.no_tune:
+        from elastic_fm_vp_vs_rho_amd import autotune
+        rep = autotune.tune(prop, use_cache=False, verbose=True)
+        tiles = {k: v["tile"] for k, v in rep.items()}
+        print(json.dumps({"autotune": rep}), file=sys.stderr)
 
     for _ in range(args.warmup):
         prop.gradient(obs, K)
@@ -163,7 +133,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": [wl.nx, wl.ny, wl.nz], "nt": nt, "shots": wl.S,
                    "receivers": int(wl.rec.shape[1]), "checkpoint_interval": K,
-                   "gradient": "exact discrete adjoint, sqrt(nt) checkpoints"},
+                   "gradient": "exact discrete adjoint, sqrt(nt) checkpoints", "tiles": tiles},
         "s_per_gradient": s_per_grad,
         "forward_grid_pt_updates_per_s": None,
         "misfit": J,

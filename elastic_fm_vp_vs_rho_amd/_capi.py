@@ -48,8 +48,9 @@ P = ctypes.POINTER
 _SIGS = {
     "efm_version": ([], c_int),
     "efm_last_error": ([ctypes.c_char_p, c_int], c_int),
-    "efm_set_tile": ([c_int, c_int, c_int, c_int], c_int),
-    "efm_get_tile": ([c_int, P(c_int), P(c_int), P(c_int)], c_int),
+    "efm_set_tile": ([c_int, c_int, c_int, c_int, c_int], c_int),
+    "efm_get_tile": ([c_int, P(c_int), P(c_int), P(c_int), P(c_int)], c_int),
+    "efm_component_pad": ([], c_int),
     "efm_tile_variants": ([P(c_int), c_int], c_int),
     "efm_model_lame": ([c_void_p] * 5 + [c_ll, c_void_p], c_int),
     "efm_model_lame_adjoint": ([c_void_p] * 9 + [c_ll, c_void_p], c_int),

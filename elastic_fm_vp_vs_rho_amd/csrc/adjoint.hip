@@ -37,8 +37,8 @@ struct AdjArgs {
 // ---------------------------------------------------------------------------
 // R2: transpose of the velocity sweep
 // ---------------------------------------------------------------------------
-template <int BX, int BZ, int NYC>
-__global__ __launch_bounds__(BX *BZ) void k_adj_velocity(Geo g, AdjArgs A, int n) {
+template <int BX, int BZ, int NYC, int OCC>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_velocity(Geo g, AdjArgs A, int n) {
     const int x = blockIdx.x * BX + threadIdx.x;
     const int z = blockIdx.y * BZ + threadIdx.y;
     const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(BX *BZ) void k_adj_velocity(Geo g, AdjArgs A, int n
     const float azm = szm >= 0 ? azs[z - 1] : 0.f, azp = szp >= 0 ? azs[z + 1] : 0.f;
     const long long csx = px_cstride(g), csy = py_cstride(g), csz = pz_cstride(g);
 
-    const float *bbx = A.bbar, *bby = A.bbar + N, *bbz = A.bbar + 2 * N;
+    const float *bbx = A.bbar, *bby = A.bbar + g.ps, *bbz = A.bbar + 2 * g.ps;
     const float *vbx = A.a.v[0], *vby = A.a.v[1], *vbz = A.a.v[2];
 
     for (int y = y0; y < y1; ++y) {
@@ -184,18 +184,18 @@ __global__ __launch_bounds__(BX *BZ) void k_adj_velocity(Geo g, AdjArgs A, int n
             f_yzz += bz0 * A.fn.pz[pi + csz] + az0 * h_yzz;
             f_zz += bz0 * A.fn.pz[pi + 2 * csz] + az0 * h_zz;
         }
-        float *acc = A.acc + (long long)s * 8 * N + lc;
-        acc[5 * N] += vbx[t] * (f_xx + f_xyy + f_xzz);
-        acc[6 * N] += vby[t] * (f_xyx + f_yy + f_yzz);
-        acc[7 * N] += vbz[t] * (f_xzx + f_yzy + f_zz);
+        float *acc = A.acc + (long long)s * 8 * g.ps + lc;
+        acc[5 * g.ps] += vbx[t] * (f_xx + f_xyy + f_xzz);
+        acc[6 * g.ps] += vby[t] * (f_xyx + f_yy + f_yzz);
+        acc[7 * g.ps] += vbz[t] * (f_xzx + f_yzy + f_zz);
     }
 }
 
 // ---------------------------------------------------------------------------
 // R4: transpose of the stress sweep
 // ---------------------------------------------------------------------------
-template <int BX, int BZ, int NYC>
-__global__ __launch_bounds__(BX *BZ) void k_adj_stress(Geo g, AdjArgs A, int n) {
+template <int BX, int BZ, int NYC, int OCC>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_stress(Geo g, AdjArgs A, int n) {
     const int x = blockIdx.x * BX + threadIdx.x;
     const int z = blockIdx.y * BZ + threadIdx.y;
     const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BX *BZ) void k_adj_stress(Geo g, AdjArgs A, int n) 
     const float azm = szm >= 0 ? azs[z - 1] : 0.f, azp = szp >= 0 ? azs[z + 1] : 0.f;
     const long long csx = px_cstride(g), csy = py_cstride(g), csz = pz_cstride(g);
 
-    const float *mxy = A.mbar, *mxz = A.mbar + N, *myz = A.mbar + 2 * N;
+    const float *mxy = A.mbar, *mxz = A.mbar + g.ps, *myz = A.mbar + 2 * g.ps;
     const float *lam = A.lam, *mu = A.mu;
     const float *Txx = A.a.t[0], *Tyy = A.a.t[1], *Tzz = A.a.t[2];
     const float *Txy = A.a.t[3], *Txz = A.a.t[4], *Tyz = A.a.t[5];
@@ -356,12 +356,12 @@ __global__ __launch_bounds__(BX *BZ) void k_adj_stress(Geo g, AdjArgs A, int n) 
             fyz += bz0 * A.fn.pz[pi + csz] + az0 * d_yz;
             fzz += bz0 * A.fn.pz[pi + 2 * csz] + az0 * d_zz;
         }
-        float *acc = A.acc + (long long)s * 8 * N + lc;
+        float *acc = A.acc + (long long)s * 8 * g.ps + lc;
         acc[0] += (txx + tyy + tzz) * (fxx + fyy + fzz);
-        acc[N] += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
-        acc[2 * N] += txy * (fxy + fyx);
-        acc[3 * N] += txz * (fxz + fzx);
-        acc[4 * N] += tyz * (fyz + fzy);
+        acc[g.ps] += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
+        acc[2 * g.ps] += txy * (fxy + fyx);
+        acc[3 * g.ps] += txz * (fxz + fzx);
+        acc[4 * g.ps] += tyz * (fyz + fzy);
     }
 }
 
@@ -414,8 +414,8 @@ __global__ void k_derived(Geo g, const float *mu, const float *rho, float *mbar,
         by = 2.f / (r + rho[i + sxz]);
         bz = 2.f / (r + rho[i + nx]);
     }
-    mbar[i] = mxy; mbar[i + g.N] = mxz; mbar[i + 2 * g.N] = myz;
-    bbar[i] = bx; bbar[i + g.N] = by; bbar[i + 2 * g.N] = bz;
+    mbar[i] = mxy; mbar[i + g.ps] = mxz; mbar[i + 2 * g.ps] = myz;
+    bbar[i] = bx; bbar[i + g.ps] = by; bbar[i + 2 * g.ps] = bz;
 }
 
 __global__ void k_combine(Geo g, const float *mu, const float *mbar, const float *bbar,
@@ -425,7 +425,7 @@ __global__ void k_combine(Geo g, const float *mu, const float *mbar, const float
     const int x = (int)(i % g.nx);
     const int z = (int)((i / g.nx) % g.nz);
     const int y = (int)(i / g.sxz);
-    const long long N = g.N, nx = g.nx, sxz = g.sxz;
+    const long long N = g.ps, nx = g.nx, sxz = g.sxz;   // N: padded component stride
     const float c = g.c;
     const float *mxy = mbar, *mxz = mbar + N, *myz = mbar + 2 * N;
     const float *bbx = bbar, *bby = bbar + N, *bbz = bbar + 2 * N;
@@ -481,33 +481,11 @@ __global__ void k_lame_adjoint(const float *vp, const float *vs, const float *rh
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-#define EFM_TILES(X)  \
-    X(64, 4, 1)       \
-    X(64, 4, 4)       \
-    X(64, 4, 8)       \
-    X(64, 2, 8)       \
-    X(64, 8, 4)       \
-    X(64, 4, 16)      \
-    X(128, 2, 4)      \
-    X(64, 1, 16)
-
-static inline dim3 sweep_grid(const Geo &g, const Tile &t) {
-    const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
-    return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
-}
-
-#define LAUNCH_IF(BX_, BZ_, NY_, KER, ...)                                                     \
-    if (!launched && t.bx == BX_ && t.bz == BZ_ && t.ny == NY_) {                               \
-        hipLaunchKernelGGL((KER<BX_, BZ_, NY_>), sweep_grid(g, t), dim3(BX_, BZ_), 0, st,       \
-                           __VA_ARGS__);                                                        \
-        launched = true;                                                                       \
-    }
-
 int launch_adj_velocity(const Geo &g, const AdjArgs &A, int n, hipStream_t st) {
     const Tile t = get_tile(2);
     bool launched = false;
-#define X(a, b, c) LAUNCH_IF(a, b, c, k_adj_velocity, g, A, n)
-    EFM_TILES(X)
+#define X(a, b, c, o) EFM_LAUNCH_IF(a, b, c, o, k_adj_velocity, g, A, n)
+    EFM_TILE_LIST(X)
 #undef X
     if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported adjoint-velocity tile");
     return check_launch("adjoint velocity sweep");
@@ -516,8 +494,8 @@ int launch_adj_velocity(const Geo &g, const AdjArgs &A, int n, hipStream_t st) {
 int launch_adj_stress(const Geo &g, const AdjArgs &A, int n, hipStream_t st) {
     const Tile t = get_tile(3);
     bool launched = false;
-#define X(a, b, c) LAUNCH_IF(a, b, c, k_adj_stress, g, A, n)
-    EFM_TILES(X)
+#define X(a, b, c, o) EFM_LAUNCH_IF(a, b, c, o, k_adj_stress, g, A, n)
+    EFM_TILE_LIST(X)
 #undef X
     if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported adjoint-stress tile");
     return check_launch("adjoint stress sweep");

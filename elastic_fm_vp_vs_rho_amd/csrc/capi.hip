@@ -26,9 +26,10 @@ int check_launch(const char *what) {
 
 // Tiles: 0 stress, 1 velocity, 2 adjoint velocity, 3 adjoint stress.
 static std::mutex g_tile_mu;
-static Tile g_tiles[4] = {{64, 4, 4}, {64, 4, 4}, {64, 4, 4}, {64, 4, 4}};
-static const Tile kTiles[] = {{64, 4, 1}, {64, 4, 4}, {64, 4, 8}, {64, 2, 8},
-                              {64, 8, 4}, {64, 4, 16}, {128, 2, 4}, {64, 1, 16}};
+static Tile g_tiles[4] = {{64, 4, 4, 0}, {64, 4, 4, 0}, {64, 4, 4, 0}, {64, 4, 4, 0}};
+#define EFM_TILE_ENTRY(a, b, c, o) {a, b, c, o},
+static const Tile kTiles[] = {EFM_TILE_LIST(EFM_TILE_ENTRY)};
+#undef EFM_TILE_ENTRY
 
 Tile get_tile(int kernel) {
     std::lock_guard<std::mutex> lk(g_tile_mu);
@@ -38,7 +39,7 @@ Tile get_tile(int kernel) {
 int set_tile(int kernel, Tile t) {
     if (kernel < 0 || kernel > 3) return fail(EFM_E_ARG, "tile kernel id out of range");
     for (const Tile &k : kTiles)
-        if (k.bx == t.bx && k.bz == t.bz && k.ny == t.ny) {
+        if (k.bx == t.bx && k.bz == t.bz && k.ny == t.ny && k.occ == t.occ) {
             std::lock_guard<std::mutex> lk(g_tile_mu);
             g_tiles[kernel] = t;
             return EFM_OK;
@@ -85,14 +86,17 @@ extern "C" int efm_last_error(char *buf, int len) {
     return (int)g_err.size();
 }
 
-extern "C" int efm_set_tile(int kernel, int bx, int bz, int ny) {
-    return set_tile(kernel, Tile{bx, bz, ny});
+extern "C" int efm_component_pad(void) { return EFM_PAD; }
+
+extern "C" int efm_set_tile(int kernel, int bx, int bz, int ny, int occ) {
+    return set_tile(kernel, Tile{bx, bz, ny, occ});
 }
 
-extern "C" int efm_get_tile(int kernel, int *bx, int *bz, int *ny) {
-    if (kernel < 0 || kernel > 3 || !bx || !bz || !ny) return fail(EFM_E_ARG, "efm_get_tile: bad argument");
+extern "C" int efm_get_tile(int kernel, int *bx, int *bz, int *ny, int *occ) {
+    if (kernel < 0 || kernel > 3 || !bx || !bz || !ny || !occ)
+        return fail(EFM_E_ARG, "efm_get_tile: bad argument");
     const Tile t = get_tile(kernel);
-    *bx = t.bx; *bz = t.bz; *ny = t.ny;
+    *bx = t.bx; *bz = t.bz; *ny = t.ny; *occ = t.occ;
     return EFM_OK;
 }
 
@@ -100,7 +104,8 @@ extern "C" int efm_tile_variants(int *out, int max) {
     const int n = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
     if (out)
         for (int i = 0; i < n && i < max; ++i) {
-            out[3 * i] = kTiles[i].bx; out[3 * i + 1] = kTiles[i].bz; out[3 * i + 2] = kTiles[i].ny;
+            out[4 * i] = kTiles[i].bx; out[4 * i + 1] = kTiles[i].bz;
+            out[4 * i + 2] = kTiles[i].ny; out[4 * i + 3] = kTiles[i].occ;
         }
     return n;
 }

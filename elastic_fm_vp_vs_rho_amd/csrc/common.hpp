@@ -21,6 +21,7 @@ struct Geo {
     float c;                           // signed dt/dx
     long long N;                       // nx*ny*nz
     long long sxz;                     // nx*nz (y stride)
+    long long ps;                      // padded component stride N + EFM_PAD (acc, mbar, bbar)
 };
 
 struct StatePtrs {
@@ -47,6 +48,7 @@ inline Geo make_geo(const EfmDims &d) {
     g.ntmax = d.ntmax; g.nrec = d.nrec; g.c = d.dtdx;
     g.sxz = (long long)d.nx * d.nz;
     g.N = g.sxz * d.ny;
+    g.ps = g.N + EFM_PAD;
     return g;
 }
 
@@ -103,7 +105,38 @@ int check_launch(const char *what);
 // ---- tile configuration chosen by the autotuner ------------------------------
 struct Tile {
     int bx, bz, ny;  // threads in x, threads in z, cells per thread along y
+    int occ;         // minimum waves per SIMD the compiler must allow (0: its own choice)
 };
+
+// Compiled tile variants (bx, bz, ny, occ).  Whole 64-lane waves along x.
+#define EFM_TILE_LIST(X) \
+    X(64, 4, 1, 0)       \
+    X(64, 4, 4, 0)       \
+    X(64, 4, 8, 0)       \
+    X(64, 2, 8, 0)       \
+    X(64, 8, 4, 0)       \
+    X(64, 4, 16, 0)      \
+    X(128, 2, 4, 0)      \
+    X(64, 1, 16, 0)      \
+    X(64, 4, 1, 4)       \
+    X(64, 4, 4, 4)       \
+    X(64, 2, 8, 4)       \
+    X(64, 8, 2, 4)
+
+inline dim3 sweep_grid(const Geo &g, const Tile &t) {
+    const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
+    return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
+}
+
+// Launch KER<bx,bz,ny,occ> for the selected tile; sets `launched`.
+#define EFM_LAUNCH_IF(BX_, BZ_, NY_, OCC_, KER, ...)                                           \
+    if (!launched && t.bx == BX_ && t.bz == BZ_ && t.ny == NY_ && t.occ == OCC_) {              \
+        hipLaunchKernelGGL((KER<BX_, BZ_, NY_, OCC_>), sweep_grid(g, t), dim3(BX_, BZ_), 0, st, \
+                           __VA_ARGS__);                                                        \
+        launched = true;                                                                       \
+    }
+#define EFM_KERNEL_ATTRS(BX, BZ, OCC) \
+    __launch_bounds__(BX *BZ) __attribute__((amdgpu_waves_per_eu((OCC) > 0 ? (OCC) : 1)))
 Tile get_tile(int kernel);  // 0 stress, 1 velocity, 2 adjoint-velocity, 3 adjoint-stress
 int set_tile(int kernel, Tile t);
 

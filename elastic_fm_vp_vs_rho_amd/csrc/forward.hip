@@ -25,8 +25,8 @@ struct Med {
 // ---------------------------------------------------------------------------
 // stress sweep
 // ---------------------------------------------------------------------------
-template <int BX, int BZ, int NYC>
-__global__ __launch_bounds__(BX *BZ) void k_stress(Geo g, Med m, ShotPtrs sh, StatePtrs in,
+template <int BX, int BZ, int NYC, int OCC>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh, StatePtrs in,
                                                   StatePtrs out, int it, int src_mode) {
     const int x = blockIdx.x * BX + threadIdx.x;
     const int z = blockIdx.y * BZ + threadIdx.y;
@@ -138,8 +138,8 @@ __global__ __launch_bounds__(BX *BZ) void k_stress(Geo g, Med m, ShotPtrs sh, St
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
-template <int BX, int BZ, int NYC>
-__global__ __launch_bounds__(BX *BZ) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
+template <int BX, int BZ, int NYC, int OCC>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
                                                     StatePtrs out, int it) {
     const int x = blockIdx.x * BX + threadIdx.x;
     const int z = blockIdx.y * BZ + threadIdx.y;
@@ -300,35 +300,13 @@ __global__ void k_border(Geo g, ShotPtrs sh, StatePtrs st, float *borders, int W
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-#define EFM_TILES(X)  \
-    X(64, 4, 1)       \
-    X(64, 4, 4)       \
-    X(64, 4, 8)       \
-    X(64, 2, 8)       \
-    X(64, 8, 4)       \
-    X(64, 4, 16)      \
-    X(128, 2, 4)      \
-    X(64, 1, 16)
-
-static inline dim3 sweep_grid(const Geo &g, const Tile &t) {
-    const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
-    return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
-}
-
-#define LAUNCH_IF(BX_, BZ_, NY_, KER, ...)                                                     \
-    if (!launched && t.bx == BX_ && t.bz == BZ_ && t.ny == NY_) {                               \
-        hipLaunchKernelGGL((KER<BX_, BZ_, NY_>), sweep_grid(g, t), dim3(BX_, BZ_), 0, st,       \
-                           __VA_ARGS__);                                                        \
-        launched = true;                                                                       \
-    }
-
 int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
                       const StatePtrs &out, int it, int src_mode, hipStream_t st) {
     const Tile t = get_tile(0);
     Med md{m.lam, m.mu, m.rho};
     bool launched = false;
-#define X(a, b, c) LAUNCH_IF(a, b, c, k_stress, g, md, sh, in, out, it, src_mode)
-    EFM_TILES(X)
+#define X(a, b, c, o) EFM_LAUNCH_IF(a, b, c, o, k_stress, g, md, sh, in, out, it, src_mode)
+    EFM_TILE_LIST(X)
 #undef X
     if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported stress tile");
     return check_launch("stress sweep");
@@ -344,8 +322,8 @@ int launch_velocity(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const 
     const Tile t = get_tile(1);
     Med md{m.lam, m.mu, m.rho};
     bool launched = false;
-#define X(a, b, c) LAUNCH_IF(a, b, c, k_velocity, g, md, sh, in, out, it)
-    EFM_TILES(X)
+#define X(a, b, c, o) EFM_LAUNCH_IF(a, b, c, o, k_velocity, g, md, sh, in, out, it)
+    EFM_TILE_LIST(X)
 #undef X
     if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported velocity tile");
     return check_launch("velocity sweep");

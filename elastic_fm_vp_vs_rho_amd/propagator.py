@@ -112,8 +112,22 @@ class ShotBatch:
                         _ptr(self.nt), _ptr(self.rec))
 
 
+def component_pad():
+    return int(_capi.lib().efm_component_pad())
+
+
+def padded_components(ncomp, N, device, pad=None):
+    """(ncomp, N) float32 zeros whose component stride is N + pad (the layout of
+    the library's multi-component scratch arrays)."""
+    pad = component_pad() if pad is None else pad
+    buf = torch.zeros(ncomp * (N + pad), dtype=torch.float32, device=device)
+    return torch.as_strided(buf, (ncomp, N), (N + pad, 1))
+
+
 class Wavefield:
-    """Batched state: 9 volumes (S, ny, nz, nx) in one tensor + three ψ slabs."""
+    """Batched state: 9 volumes (S, ny, nz, nx) + three ψ slabs.  The 9 volumes
+    share one allocation with a padded field stride (S·N + pad floats), so that
+    equal cells of different fields are not a power of two apart."""
 
     def __init__(self, grid: Grid, S: int, device="cuda", psi_only_of=None):
         self.grid, self.S = grid, S
@@ -121,7 +135,11 @@ class Wavefield:
         ny, nz, nx = grid.shape
         dev = torch.device(device)
         if psi_only_of is None:
-            self.fields = torch.zeros((9, S, ny, nz, nx), dtype=torch.float32, device=dev)
+            per = S * grid.N
+            pad = 1088
+            buf = torch.zeros(9 * (per + pad), dtype=torch.float32, device=dev)
+            self.fields = torch.as_strided(buf, (9, S, ny, nz, nx),
+                                           (per + pad, grid.N, nz * nx, nx, 1))
         else:
             self.fields = psi_only_of.fields
         self.psi_x = torch.zeros((S, 6, ny, nz, Lx), dtype=torch.float32, device=dev)
@@ -148,11 +166,8 @@ class Wavefield:
         return self
 
     def clone(self):
-        w = Wavefield.__new__(Wavefield)
-        w.grid, w.S, w._s = self.grid, self.S, None
-        w.fields, w.psi_x = self.fields.clone(), self.psi_x.clone()
-        w.psi_y, w.psi_z = self.psi_y.clone(), self.psi_z.clone()
-        return w
+        w = Wavefield(self.grid, self.S, self.fields.device)
+        return w.copy_(self)
 
     def nbytes(self):
         return sum(t.numel() * 4 for t in (self.fields, self.psi_x, self.psi_y, self.psi_z))
@@ -245,15 +260,15 @@ class Propagator:
         if tuple(resid.shape) != (s.S, 3, s.ntmax, s.nrec):
             raise ValueError(f"residual must be {(s.S, 3, s.ntmax, s.nrec)}")
         d, m, sh = self.dims(), self.medium(), s.struct()
-        mbar = torch.empty((3,) + g.shape, dtype=torch.float32, device=dev)
-        bbar = torch.empty_like(mbar)
+        mbar = padded_components(3, g.N, dev)
+        bbar = padded_components(3, g.N, dev)
         check(self.lib.efm_derived_medium(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
                                           _stream()), "efm_derived_medium")
         nt = s.ntmax
         bufs = [self.new_wavefield() for _ in range(min(K, nt) + 1)]
         adj = self.new_wavefield()
         adj2 = Wavefield(g, s.S, dev, psi_only_of=adj)
-        acc = torch.zeros((s.S, 8) + g.shape, dtype=torch.float32, device=dev)
+        acc = padded_components(s.S * 8, g.N, dev)
         gm6 = torch.zeros((s.S, 6), dtype=torch.float64, device=dev)
         for seg in range(len(snaps) - 1, -1, -1):
             n0 = seg * K

@@ -31,6 +31,12 @@
 extern "C" {
 #endif
 
+/* Component stride padding (floats) of the multi-component scratch arrays
+ * (adjoint accumulators, derived medium): component k of such an array starts
+ * at k*(N + EFM_PAD), N = nx*ny*nz, so that equal-index elements of different
+ * components do not fall on power-of-two address strides. */
+#define EFM_PAD 1088
+
 #define EFM_OK 0
 #define EFM_E_ARG (-1)
 #define EFM_E_HIP (-2)
@@ -73,11 +79,14 @@ int efm_last_error(char *buf, int len);
 /* ---- tile selection (no reference counterpart: the reference hard-codes
  * block=(32,2,4), M:2764-2766).  kernel: 0 stress, 1 velocity, 2 adjoint
  * velocity, 3 adjoint stress.  A tile is bx × bz threads, ny cells per thread
- * along y.  efm_tile_variants writes the compiled tiles as triples and returns
- * their count.  The autotuner (autotune.py) drives these. */
-int efm_set_tile(int kernel, int bx, int bz, int ny);
-int efm_get_tile(int kernel, int *bx, int *bz, int *ny);
+ * along y; occ is the minimum waves per SIMD the kernel was compiled to allow
+ * (0 = compiler's choice).  efm_tile_variants writes the compiled tiles as
+ * quadruples and returns their count.  The autotuner (autotune.py) drives these. */
+int efm_set_tile(int kernel, int bx, int bz, int ny, int occ);
+int efm_get_tile(int kernel, int *bx, int *bz, int *ny, int *occ);
 int efm_tile_variants(int *out, int max);
+/* EFM_PAD as compiled into the library. */
+int efm_component_pad(void);
 
 /* ---- model maps -------------------------------------------------------- */
 /* Model.compute_Lambda_Mu (M:1196-1198): mu = vs*vs*rho, lam = vp*vp*rho - 2*mu,
@@ -119,16 +128,17 @@ int efm_forward_states(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
                        void *stream);
 
 /* ---- exact discrete adjoint -------------------------------------------- */
-/* Derived medium for the adjoint: mbar (3, ny, nz, nx) = harmonic-mean μ_xy,
- * μ_xz, μ_yz; bbar (3, ...) = 2/(ρ+ρ') along x, y, z.  Interior only. */
+/* Derived medium for the adjoint: mbar (3 components) = harmonic-mean μ_xy,
+ * μ_xz, μ_yz; bbar (3) = 2/(ρ+ρ') along x, y, z; component stride N + EFM_PAD.
+ * Interior only (zero elsewhere). */
 int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float *bbar,
                        void *stream);
 /* Reverse sweep of steps n = n1-1 down to n0 (n0 < n1).  fwd[k] is the
  * forward pre-step state S^(n0+k), k = 0 .. n1-n0 (so fwd[n1-n0] is S^n1).
  * adj/adj2 are the adjoint state and a second ψ buffer (ping-pong; on return
  * the live ψ adjoint is in adj).  resid is ∂J/∂gathers (S, 3, ntmax, nrec).
- * acc (S, 8, ny, nz, nx) accumulates λ, μ_n, μ_xy, μ_xz, μ_yz, ρx, ρy, ρz
- * sums; gm6 (S, 6) double accumulates ∂J/∂M.  `last` = 1 when n0 is the
+ * acc (S·8 components of stride N + EFM_PAD) accumulates λ, μ_n, μ_xy, μ_xz,
+ * μ_yz, ρx, ρy, ρz sums; gm6 (S, 6) double accumulates ∂J/∂M.  `last` = 1 when n0 is the
  * first step of the run (closes the moment-tensor sum of step n0). */
 int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mbar, const float *bbar,
                 const EfmShots *sh, const EfmState *const *fwd, const EfmState *adj,

@@ -43,12 +43,13 @@ def test_version_and_errors(L):
     rc = L.efm_forward(None, None, None, None, None, None, 0, 0, 1, None)
     assert rc == -1
     assert "null" in _capi.last_error()
-    assert L.efm_set_tile(0, 48, 3, 5) == -3
+    assert L.efm_set_tile(0, 48, 3, 5, 0) == -3
     assert "tile" in _capi.last_error()
     n = L.efm_tile_variants(None, 0)
-    buf = (ctypes.c_int * (3 * n))()
+    buf = (ctypes.c_int * (4 * n))()
     assert L.efm_tile_variants(buf, n) == n
-    assert all(buf[3 * i] % 64 == 0 for i in range(n))   # whole 64-lane waves along x
+    assert all(buf[4 * i] % 64 == 0 for i in range(n))   # whole 64-lane waves along x
+    assert L.efm_component_pad() == 1088
 
 
 def test_argument_validation_without_gpu(L):
