@@ -48,9 +48,8 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_velocity(Geo g, AdjArgs A, i
     if (n >= A.sh.nt[s]) return;
     const int y0 = H + yb * NYC;
     const int y1 = min(y0 + NYC, g.ny - H);
-    const int nx = g.nx;
-    const long long sxz = g.sxz, N = g.N;
-    const long long so = (long long)s * N;
+    const int nx = g.nx, sxz = g.sxz32;
+    const long long so = (long long)s * g.N;
     const float c = g.c;
     const ShotPtrs &sh = A.sh;
 
@@ -68,14 +67,41 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_velocity(Geo g, AdjArgs A, i
     const float axm = sxm >= 0 ? axs[x - 1] : 0.f, axp = sxp >= 0 ? axs[x + 1] : 0.f;
     const float az0 = sz0 >= 0 ? azs[z] : 0.f, bz0 = sz0 >= 0 ? bzs[z] : 0.f;
     const float azm = szm >= 0 ? azs[z - 1] : 0.f, azp = szp >= 0 ? azs[z + 1] : 0.f;
-    const long long csx = px_cstride(g), csy = py_cstride(g), csz = pz_cstride(g);
+    const int csx = g.pxc, csy = g.pyc, csz = g.pzc;
 
-    const float *bbx = A.bbar, *bby = A.bbar + g.ps, *bbz = A.bbar + 2 * g.ps;
-    const float *vbx = A.a.v[0], *vby = A.a.v[1], *vbz = A.a.v[2];
+    const float *__restrict__ bbx = A.bbar;
+    const float *__restrict__ bby = A.bbar + g.ps;
+    const float *__restrict__ bbz = A.bbar + 2 * g.ps;
+    const float *__restrict__ vbx = A.a.v[0] + so;
+    const float *__restrict__ vby = A.a.v[1] + so;
+    const float *__restrict__ vbz = A.a.v[2] + so;
+    float *__restrict__ tb0 = A.a.t[0] + so;
+    float *__restrict__ tb1 = A.a.t[1] + so;
+    float *__restrict__ tb2 = A.a.t[2] + so;
+    float *__restrict__ tb3 = A.a.t[3] + so;
+    float *__restrict__ tb4 = A.a.t[4] + so;
+    float *__restrict__ tb5 = A.a.t[5] + so;
+    const float *__restrict__ T0 = A.fn1.t[0] + so;
+    const float *__restrict__ T1 = A.fn1.t[1] + so;
+    const float *__restrict__ T2 = A.fn1.t[2] + so;
+    const float *__restrict__ T3 = A.fn1.t[3] + so;
+    const float *__restrict__ T4 = A.fn1.t[4] + so;
+    const float *__restrict__ T5 = A.fn1.t[5] + so;
+    const float *__restrict__ Cx = A.a.px + px_shot(g, s);
+    const float *__restrict__ Cy = A.a.py + py_shot(g, s);
+    const float *__restrict__ Cz = A.a.pz + pz_shot(g, s);
+    float *__restrict__ Cxo = A.a2.px + px_shot(g, s);
+    float *__restrict__ Cyo = A.a2.py + py_shot(g, s);
+    float *__restrict__ Czo = A.a2.pz + pz_shot(g, s);
+    const float *__restrict__ Fx = A.fn.px + px_shot(g, s);
+    const float *__restrict__ Fy = A.fn.py + py_shot(g, s);
+    const float *__restrict__ Fz = A.fn.pz + pz_shot(g, s);
+    float *__restrict__ acc5 = A.acc + ((long long)s * 8 + 5) * g.ps;
+    float *__restrict__ acc6 = A.acc + ((long long)s * 8 + 6) * g.ps;
+    float *__restrict__ acc7 = A.acc + ((long long)s * 8 + 7) * g.ps;
 
     for (int y = y0; y < y1; ++y) {
-        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
-        const long long t = so + lc;
+        const int t = x + z * nx + y * sxz;
         const int sy0 = slab_coord(y, g.ny, g.wyl, g.wyh);
         const int sym = slab_coord(y - 1, g.ny, g.wyl, g.wyh);
         const int syp = slab_coord(y + 1, g.ny, g.wyl, g.wyh);
@@ -83,18 +109,19 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_velocity(Geo g, AdjArgs A, i
         const float aym = sym >= 0 ? ays[y - 1] : 0.f, ayp = syp >= 0 ? ays[y + 1] : 0.f;
 
         // u = c·b̄·v̄ at p and at the neighbours the transposes need
-        const float ux = c * bbx[lc] * vbx[t];
-        const float uy = c * bby[lc] * vby[t];
-        const float uz = c * bbz[lc] * vbz[t];
-        const float ux_mx = c * bbx[lc - 1] * vbx[t - 1];        // txx_x at p-x
-        const float uy_my = c * bby[lc - sxz] * vby[t - sxz];    // tyy_y at p-y
-        const float uz_mz = c * bbz[lc - nx] * vbz[t - nx];      // tzz_z at p-z
-        const float ux_py = c * bbx[lc + sxz] * vbx[t + sxz];    // txy_y at p+y
-        const float uy_px = c * bby[lc + 1] * vby[t + 1];        // txy_x at p+x
-        const float ux_pz = c * bbx[lc + nx] * vbx[t + nx];      // txz_z at p+z
-        const float uz_px = c * bbz[lc + 1] * vbz[t + 1];        // txz_x at p+x
-        const float uy_pz = c * bby[lc + nx] * vby[t + nx];      // tyz_z at p+z
-        const float uz_py = c * bbz[lc + sxz] * vbz[t + sxz];    // tyz_y at p+y
+        const float vx0 = vbx[t], vy0 = vby[t], vz0 = vbz[t];
+        const float ux = c * bbx[t] * vx0;
+        const float uy = c * bby[t] * vy0;
+        const float uz = c * bbz[t] * vz0;
+        const float ux_mx = c * bbx[t - 1] * vbx[t - 1];        // txx_x at p-x
+        const float uy_my = c * bby[t - sxz] * vby[t - sxz];    // tyy_y at p-y
+        const float uz_mz = c * bbz[t - nx] * vbz[t - nx];      // tzz_z at p-z
+        const float ux_py = c * bbx[t + sxz] * vbx[t + sxz];    // txy_y at p+y
+        const float uy_px = c * bby[t + 1] * vby[t + 1];        // txy_x at p+x
+        const float ux_pz = c * bbx[t + nx] * vbx[t + nx];      // txz_z at p+z
+        const float uz_px = c * bbz[t + 1] * vbz[t + 1];        // txz_x at p+x
+        const float uy_pz = c * bby[t + nx] * vby[t + nx];      // tyz_z at p+z
+        const float uz_py = c * bbz[t + sxz] * vbz[t + sxz];    // tyz_y at p+y
 
         // h̄ = u + a(C + u) with C read from the "in" buffer
         float hb_xx_m = ux_mx, hb_xx_0 = ux;           // txx_x
@@ -106,88 +133,84 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_velocity(Geo g, AdjArgs A, i
         float hb_xzx_0 = uz, hb_xzx_p = uz_px;         // txz_x
         float hb_yzz_0 = uy, hb_yzz_p = uy_pz;         // tyz_z
         float hb_yzy_0 = uz, hb_yzy_p = uz_py;         // tyz_y
-        const float *Cx = A.a.px, *Cy = A.a.py, *Cz = A.a.pz;
-        float *Cxo = A.a2.px, *Cyo = A.a2.py, *Czo = A.a2.pz;
         // x-slab components 3 txx_x, 4 txy_x, 5 txz_x
-        if (sxm >= 0) hb_xx_m += axm * (Cx[px_index(g, s, 3, y, z, sxm)] + ux_mx);
+        if (sxm >= 0) hb_xx_m += axm * (Cx[px_off(g, y, z, sxm) + 3 * csx] + ux_mx);
         if (sxp >= 0) {
-            const long long pi = px_index(g, s, 4, y, z, sxp);
+            const int pi = px_off(g, y, z, sxp) + 4 * csx;
             hb_xyx_p += axp * (Cx[pi] + uy_px);
             hb_xzx_p += axp * (Cx[pi + csx] + uz_px);
         }
         if (sx0 >= 0) {
-            const long long pi = px_index(g, s, 3, y, z, sx0);
+            const int pi = px_off(g, y, z, sx0) + 3 * csx;
             const float c0 = Cx[pi] + ux, c1 = Cx[pi + csx] + uy, c2 = Cx[pi + 2 * csx] + uz;
             hb_xx_0 += ax0 * c0; hb_xyx_0 += ax0 * c1; hb_xzx_0 += ax0 * c2;
             Cxo[pi] = bx0 * c0; Cxo[pi + csx] = bx0 * c1; Cxo[pi + 2 * csx] = bx0 * c2;
         }
         // y-slab components 3 txy_y, 4 tyy_y, 5 tyz_y
-        if (sym >= 0) hb_yy_m += aym * (Cy[py_index(g, s, 4, sym, z, x)] + uy_my);
+        if (sym >= 0) hb_yy_m += aym * (Cy[py_off(g, sym, z, x) + 4 * csy] + uy_my);
         if (syp >= 0) {
-            const long long pi = py_index(g, s, 3, syp, z, x);
+            const int pi = py_off(g, syp, z, x) + 3 * csy;
             hb_xyy_p += ayp * (Cy[pi] + ux_py);
             hb_yzy_p += ayp * (Cy[pi + 2 * csy] + uz_py);
         }
         if (sy0 >= 0) {
-            const long long pi = py_index(g, s, 3, sy0, z, x);
+            const int pi = py_off(g, sy0, z, x) + 3 * csy;
             const float c0 = Cy[pi] + ux, c1 = Cy[pi + csy] + uy, c2 = Cy[pi + 2 * csy] + uz;
             hb_xyy_0 += ay0 * c0; hb_yy_0 += ay0 * c1; hb_yzy_0 += ay0 * c2;
             Cyo[pi] = by0 * c0; Cyo[pi + csy] = by0 * c1; Cyo[pi + 2 * csy] = by0 * c2;
         }
         // z-slab components 3 txz_z, 4 tyz_z, 5 tzz_z
-        if (szm >= 0) hb_zz_m += azm * (Cz[pz_index(g, s, 5, y, szm, x)] + uz_mz);
+        if (szm >= 0) hb_zz_m += azm * (Cz[pz_off(g, y, szm, x) + 5 * csz] + uz_mz);
         if (szp >= 0) {
-            const long long pi = pz_index(g, s, 3, y, szp, x);
+            const int pi = pz_off(g, y, szp, x) + 3 * csz;
             hb_xzz_p += azp * (Cz[pi] + ux_pz);
             hb_yzz_p += azp * (Cz[pi + csz] + uy_pz);
         }
         if (sz0 >= 0) {
-            const long long pi = pz_index(g, s, 3, y, sz0, x);
+            const int pi = pz_off(g, y, sz0, x) + 3 * csz;
             const float c0 = Cz[pi] + ux, c1 = Cz[pi + csz] + uy, c2 = Cz[pi + 2 * csz] + uz;
             hb_xzz_0 += az0 * c0; hb_yzz_0 += az0 * c1; hb_zz_0 += az0 * c2;
             Czo[pi] = bz0 * c0; Czo[pi + csz] = bz0 * c1; Czo[pi + 2 * csz] = bz0 * c2;
         }
         // τ̄ += D_τᵀ h̄
-        A.a.t[0][t] += hb_xx_m - hb_xx_0;
-        A.a.t[1][t] += hb_yy_m - hb_yy_0;
-        A.a.t[2][t] += hb_zz_m - hb_zz_0;
-        A.a.t[3][t] += (hb_xyy_0 - hb_xyy_p) + (hb_xyx_0 - hb_xyx_p);
-        A.a.t[4][t] += (hb_xzz_0 - hb_xzz_p) + (hb_xzx_0 - hb_xzx_p);
-        A.a.t[5][t] += (hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p);
+        tb0[t] += hb_xx_m - hb_xx_0;
+        tb1[t] += hb_yy_m - hb_yy_0;
+        tb2[t] += hb_zz_m - hb_zz_0;
+        tb3[t] += (hb_xyy_0 - hb_xyy_p) + (hb_xyx_0 - hb_xyx_p);
+        tb4[t] += (hb_xzz_0 - hb_xzz_p) + (hb_xzx_0 - hb_xzx_p);
+        tb5[t] += (hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p);
 
         // ρ accumulators: forward F at p from τ^{n+1} (= τ'' of step n) and ψτ^n
-        const float *T0 = A.fn1.t[0], *T1 = A.fn1.t[1], *T2 = A.fn1.t[2];
-        const float *T3 = A.fn1.t[3], *T4 = A.fn1.t[4], *T5 = A.fn1.t[5];
         const float h_xx = T0[t + 1] - T0[t];
         const float h_yy = T1[t + sxz] - T1[t];
         const float h_zz = T2[t + nx] - T2[t];
-        const float h_xyy = T3[t] - T3[t - sxz], h_xyx = T3[t] - T3[t - 1];
-        const float h_xzz = T4[t] - T4[t - nx], h_xzx = T4[t] - T4[t - 1];
-        const float h_yzz = T5[t] - T5[t - nx], h_yzy = T5[t] - T5[t - sxz];
+        const float t3 = T3[t], t4 = T4[t], t5 = T5[t];
+        const float h_xyy = t3 - T3[t - sxz], h_xyx = t3 - T3[t - 1];
+        const float h_xzz = t4 - T4[t - nx], h_xzx = t4 - T4[t - 1];
+        const float h_yzz = t5 - T5[t - nx], h_yzy = t5 - T5[t - sxz];
         float f_xx = h_xx, f_yy = h_yy, f_zz = h_zz, f_xyy = h_xyy, f_xyx = h_xyx;
         float f_xzz = h_xzz, f_xzx = h_xzx, f_yzz = h_yzz, f_yzy = h_yzy;
         if (sx0 >= 0) {
-            const long long pi = px_index(g, s, 3, y, z, sx0);
-            f_xx += bx0 * A.fn.px[pi] + ax0 * h_xx;
-            f_xyx += bx0 * A.fn.px[pi + csx] + ax0 * h_xyx;
-            f_xzx += bx0 * A.fn.px[pi + 2 * csx] + ax0 * h_xzx;
+            const int pi = px_off(g, y, z, sx0) + 3 * csx;
+            f_xx += bx0 * Fx[pi] + ax0 * h_xx;
+            f_xyx += bx0 * Fx[pi + csx] + ax0 * h_xyx;
+            f_xzx += bx0 * Fx[pi + 2 * csx] + ax0 * h_xzx;
         }
         if (sy0 >= 0) {
-            const long long pi = py_index(g, s, 3, sy0, z, x);
-            f_xyy += by0 * A.fn.py[pi] + ay0 * h_xyy;
-            f_yy += by0 * A.fn.py[pi + csy] + ay0 * h_yy;
-            f_yzy += by0 * A.fn.py[pi + 2 * csy] + ay0 * h_yzy;
+            const int pi = py_off(g, sy0, z, x) + 3 * csy;
+            f_xyy += by0 * Fy[pi] + ay0 * h_xyy;
+            f_yy += by0 * Fy[pi + csy] + ay0 * h_yy;
+            f_yzy += by0 * Fy[pi + 2 * csy] + ay0 * h_yzy;
         }
         if (sz0 >= 0) {
-            const long long pi = pz_index(g, s, 3, y, sz0, x);
-            f_xzz += bz0 * A.fn.pz[pi] + az0 * h_xzz;
-            f_yzz += bz0 * A.fn.pz[pi + csz] + az0 * h_yzz;
-            f_zz += bz0 * A.fn.pz[pi + 2 * csz] + az0 * h_zz;
+            const int pi = pz_off(g, y, sz0, x) + 3 * csz;
+            f_xzz += bz0 * Fz[pi] + az0 * h_xzz;
+            f_yzz += bz0 * Fz[pi + csz] + az0 * h_yzz;
+            f_zz += bz0 * Fz[pi + 2 * csz] + az0 * h_zz;
         }
-        float *acc = A.acc + (long long)s * 8 * g.ps + lc;
-        acc[5 * g.ps] += vbx[t] * (f_xx + f_xyy + f_xzz);
-        acc[6 * g.ps] += vby[t] * (f_xyx + f_yy + f_yzz);
-        acc[7 * g.ps] += vbz[t] * (f_xzx + f_yzy + f_zz);
+        acc5[t] += vx0 * (f_xx + f_xyy + f_xzz);
+        acc6[t] += vy0 * (f_xyx + f_yy + f_yzz);
+        acc7[t] += vz0 * (f_xzx + f_yzy + f_zz);
     }
 }
 
@@ -205,9 +228,8 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_stress(Geo g, AdjArgs A, int
     if (n >= A.sh.nt[s]) return;
     const int y0 = H + yb * NYC;
     const int y1 = min(y0 + NYC, g.ny - H);
-    const int nx = g.nx;
-    const long long sxz = g.sxz, N = g.N;
-    const long long so = (long long)s * N;
+    const int nx = g.nx, sxz = g.sxz32;
+    const long long so = (long long)s * g.N;
     const float c = g.c;
     const ShotPtrs &sh = A.sh;
 
@@ -224,16 +246,42 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_stress(Geo g, AdjArgs A, int
     const float axm = sxm >= 0 ? axs[x - 1] : 0.f, axp = sxp >= 0 ? axs[x + 1] : 0.f;
     const float az0 = sz0 >= 0 ? azs[z] : 0.f, bz0 = sz0 >= 0 ? bzs[z] : 0.f;
     const float azm = szm >= 0 ? azs[z - 1] : 0.f, azp = szp >= 0 ? azs[z + 1] : 0.f;
-    const long long csx = px_cstride(g), csy = py_cstride(g), csz = pz_cstride(g);
+    const int csx = g.pxc, csy = g.pyc, csz = g.pzc;
 
-    const float *mxy = A.mbar, *mxz = A.mbar + g.ps, *myz = A.mbar + 2 * g.ps;
-    const float *lam = A.lam, *mu = A.mu;
-    const float *Txx = A.a.t[0], *Tyy = A.a.t[1], *Tzz = A.a.t[2];
-    const float *Txy = A.a.t[3], *Txz = A.a.t[4], *Tyz = A.a.t[5];
+    const float *__restrict__ mxy = A.mbar;
+    const float *__restrict__ mxz = A.mbar + g.ps;
+    const float *__restrict__ myz = A.mbar + 2 * g.ps;
+    const float *__restrict__ lam = A.lam;
+    const float *__restrict__ mu = A.mu;
+    const float *__restrict__ Txx = A.a.t[0] + so;
+    const float *__restrict__ Tyy = A.a.t[1] + so;
+    const float *__restrict__ Tzz = A.a.t[2] + so;
+    const float *__restrict__ Txy = A.a.t[3] + so;
+    const float *__restrict__ Txz = A.a.t[4] + so;
+    const float *__restrict__ Tyz = A.a.t[5] + so;
+    float *__restrict__ vb0 = A.a.v[0] + so;
+    float *__restrict__ vb1 = A.a.v[1] + so;
+    float *__restrict__ vb2 = A.a.v[2] + so;
+    const float *__restrict__ V0 = A.fn.v[0] + so;
+    const float *__restrict__ V1 = A.fn.v[1] + so;
+    const float *__restrict__ V2 = A.fn.v[2] + so;
+    const float *__restrict__ Cx = A.a.px + px_shot(g, s);
+    const float *__restrict__ Cy = A.a.py + py_shot(g, s);
+    const float *__restrict__ Cz = A.a.pz + pz_shot(g, s);
+    float *__restrict__ Cxo = A.a2.px + px_shot(g, s);
+    float *__restrict__ Cyo = A.a2.py + py_shot(g, s);
+    float *__restrict__ Czo = A.a2.pz + pz_shot(g, s);
+    const float *__restrict__ Fx = A.fn.px + px_shot(g, s);
+    const float *__restrict__ Fy = A.fn.py + py_shot(g, s);
+    const float *__restrict__ Fz = A.fn.pz + pz_shot(g, s);
+    float *__restrict__ acc0 = A.acc + ((long long)s * 8 + 0) * g.ps;
+    float *__restrict__ acc1 = A.acc + ((long long)s * 8 + 1) * g.ps;
+    float *__restrict__ acc2 = A.acc + ((long long)s * 8 + 2) * g.ps;
+    float *__restrict__ acc3 = A.acc + ((long long)s * 8 + 3) * g.ps;
+    float *__restrict__ acc4 = A.acc + ((long long)s * 8 + 4) * g.ps;
 
     for (int y = y0; y < y1; ++y) {
-        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
-        const long long t = so + lc;
+        const int t = x + z * nx + y * sxz;
         const int sy0 = slab_coord(y, g.ny, g.wyl, g.wyh);
         const int sym = slab_coord(y - 1, g.ny, g.wyl, g.wyh);
         const int syp = slab_coord(y + 1, g.ny, g.wyl, g.wyh);
@@ -244,35 +292,36 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_stress(Geo g, AdjArgs A, int
         const float txy = Txy[t], txz = Txz[t], tyz = Tyz[t];
 
         // ē at p
-        const float L0 = lam[lc], M0 = mu[lc];
+        const float L0 = lam[t], M0 = mu[t];
         const float e_xx = c * ((L0 + 2.f * M0) * txx + L0 * (tyy + tzz));
         const float e_yy = c * ((L0 + 2.f * M0) * tyy + L0 * (txx + tzz));
         const float e_zz = c * ((L0 + 2.f * M0) * tzz + L0 * (txx + tyy));
-        const float e_xy = c * mxy[lc] * txy;
-        const float e_xz = c * mxz[lc] * txz;
-        const float e_yz = c * myz[lc] * tyz;
+        const float e_xy = c * mxy[t] * txy;
+        const float e_xz = c * mxz[t] * txz;
+        const float e_yz = c * myz[t] * tyz;
         // ē at the neighbours
-        auto normal_xx = [&](long long o) {
-            const float L = lam[lc + o], M = mu[lc + o];
-            return c * ((L + 2.f * M) * Txx[t + o] + L * (Tyy[t + o] + Tzz[t + o]));
-        };
-        auto normal_yy = [&](long long o) {
-            const float L = lam[lc + o], M = mu[lc + o];
-            return c * ((L + 2.f * M) * Tyy[t + o] + L * (Txx[t + o] + Tzz[t + o]));
-        };
-        auto normal_zz = [&](long long o) {
-            const float L = lam[lc + o], M = mu[lc + o];
-            return c * ((L + 2.f * M) * Tzz[t + o] + L * (Txx[t + o] + Tyy[t + o]));
-        };
-        const float exx_px = normal_xx(1);                       // vx_x at p+x
-        const float eyy_py = normal_yy(sxz);                     // vy_y at p+y
-        const float ezz_pz = normal_zz(nx);                      // vz_z at p+z
-        const float exz_mz = c * mxz[lc - nx] * Txz[t - nx];     // vx_z at p-z
-        const float exy_my = c * mxy[lc - sxz] * Txy[t - sxz];   // vx_y at p-y
-        const float exy_mx = c * mxy[lc - 1] * Txy[t - 1];       // vy_x at p-x
-        const float eyz_mz = c * myz[lc - nx] * Tyz[t - nx];     // vy_z at p-z
-        const float exz_mx = c * mxz[lc - 1] * Txz[t - 1];       // vz_x at p-x
-        const float eyz_my = c * myz[lc - sxz] * Tyz[t - sxz];   // vz_y at p-y
+        float exx_px, eyy_py, ezz_pz;
+        {
+            const int o = t + 1;
+            const float L = lam[o], M = mu[o];
+            exx_px = c * ((L + 2.f * M) * Txx[o] + L * (Tyy[o] + Tzz[o]));   // vx_x at p+x
+        }
+        {
+            const int o = t + sxz;
+            const float L = lam[o], M = mu[o];
+            eyy_py = c * ((L + 2.f * M) * Tyy[o] + L * (Txx[o] + Tzz[o]));   // vy_y at p+y
+        }
+        {
+            const int o = t + nx;
+            const float L = lam[o], M = mu[o];
+            ezz_pz = c * ((L + 2.f * M) * Tzz[o] + L * (Txx[o] + Tyy[o]));   // vz_z at p+z
+        }
+        const float exz_mz = c * mxz[t - nx] * Txz[t - nx];     // vx_z at p-z
+        const float exy_my = c * mxy[t - sxz] * Txy[t - sxz];   // vx_y at p-y
+        const float exy_mx = c * mxy[t - 1] * Txy[t - 1];       // vy_x at p-x
+        const float eyz_mz = c * myz[t - nx] * Tyz[t - nx];     // vy_z at p-z
+        const float exz_mx = c * mxz[t - 1] * Txz[t - 1];       // vz_x at p-x
+        const float eyz_my = c * myz[t - sxz] * Tyz[t - sxz];   // vz_y at p-y
 
         float g_vxx_0 = e_xx, g_vxx_p = exx_px;     // vx_x
         float g_vyy_0 = e_yy, g_vyy_p = eyy_py;     // vy_y
@@ -283,54 +332,51 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_stress(Geo g, AdjArgs A, int
         float g_vyz_m = eyz_mz, g_vyz_0 = e_yz;     // vy_z
         float g_vzx_m = exz_mx, g_vzx_0 = e_xz;     // vz_x
         float g_vzy_m = eyz_my, g_vzy_0 = e_yz;     // vz_y
-        const float *Cx = A.a.px, *Cy = A.a.py, *Cz = A.a.pz;
-        float *Cxo = A.a2.px, *Cyo = A.a2.py, *Czo = A.a2.pz;
         // x-slab components 0 vx_x, 1 vy_x, 2 vz_x
-        if (sxp >= 0) g_vxx_p += axp * (Cx[px_index(g, s, 0, y, z, sxp)] + exx_px);
+        if (sxp >= 0) g_vxx_p += axp * (Cx[px_off(g, y, z, sxp)] + exx_px);
         if (sxm >= 0) {
-            const long long pi = px_index(g, s, 1, y, z, sxm);
+            const int pi = px_off(g, y, z, sxm) + csx;
             g_vyx_m += axm * (Cx[pi] + exy_mx);
             g_vzx_m += axm * (Cx[pi + csx] + exz_mx);
         }
         if (sx0 >= 0) {
-            const long long pi = px_index(g, s, 0, y, z, sx0);
+            const int pi = px_off(g, y, z, sx0);
             const float c0 = Cx[pi] + e_xx, c1 = Cx[pi + csx] + e_xy, c2 = Cx[pi + 2 * csx] + e_xz;
             g_vxx_0 += ax0 * c0; g_vyx_0 += ax0 * c1; g_vzx_0 += ax0 * c2;
             Cxo[pi] = bx0 * c0; Cxo[pi + csx] = bx0 * c1; Cxo[pi + 2 * csx] = bx0 * c2;
         }
         // y-slab components 0 vx_y, 1 vy_y, 2 vz_y
-        if (syp >= 0) g_vyy_p += ayp * (Cy[py_index(g, s, 1, syp, z, x)] + eyy_py);
+        if (syp >= 0) g_vyy_p += ayp * (Cy[py_off(g, syp, z, x) + csy] + eyy_py);
         if (sym >= 0) {
-            const long long pi = py_index(g, s, 0, sym, z, x);
+            const int pi = py_off(g, sym, z, x);
             g_vxy_m += aym * (Cy[pi] + exy_my);
             g_vzy_m += aym * (Cy[pi + 2 * csy] + eyz_my);
         }
         if (sy0 >= 0) {
-            const long long pi = py_index(g, s, 0, sy0, z, x);
+            const int pi = py_off(g, sy0, z, x);
             const float c0 = Cy[pi] + e_xy, c1 = Cy[pi + csy] + e_yy, c2 = Cy[pi + 2 * csy] + e_yz;
             g_vxy_0 += ay0 * c0; g_vyy_0 += ay0 * c1; g_vzy_0 += ay0 * c2;
             Cyo[pi] = by0 * c0; Cyo[pi + csy] = by0 * c1; Cyo[pi + 2 * csy] = by0 * c2;
         }
         // z-slab components 0 vx_z, 1 vy_z, 2 vz_z
-        if (szp >= 0) g_vzz_p += azp * (Cz[pz_index(g, s, 2, y, szp, x)] + ezz_pz);
+        if (szp >= 0) g_vzz_p += azp * (Cz[pz_off(g, y, szp, x) + 2 * csz] + ezz_pz);
         if (szm >= 0) {
-            const long long pi = pz_index(g, s, 0, y, szm, x);
+            const int pi = pz_off(g, y, szm, x);
             g_vxz_m += azm * (Cz[pi] + exz_mz);
             g_vyz_m += azm * (Cz[pi + csz] + eyz_mz);
         }
         if (sz0 >= 0) {
-            const long long pi = pz_index(g, s, 0, y, sz0, x);
+            const int pi = pz_off(g, y, sz0, x);
             const float c0 = Cz[pi] + e_xz, c1 = Cz[pi + csz] + e_yz, c2 = Cz[pi + 2 * csz] + e_zz;
             g_vxz_0 += az0 * c0; g_vyz_0 += az0 * c1; g_vzz_0 += az0 * c2;
             Czo[pi] = bz0 * c0; Czo[pi + csz] = bz0 * c1; Czo[pi + 2 * csz] = bz0 * c2;
         }
         // v̄ += D_vᵀ ḡ
-        A.a.v[0][t] += (g_vxx_0 - g_vxx_p) + (g_vxz_m - g_vxz_0) + (g_vxy_m - g_vxy_0);
-        A.a.v[1][t] += (g_vyy_0 - g_vyy_p) + (g_vyx_m - g_vyx_0) + (g_vyz_m - g_vyz_0);
-        A.a.v[2][t] += (g_vzz_0 - g_vzz_p) + (g_vzx_m - g_vzx_0) + (g_vzy_m - g_vzy_0);
+        vb0[t] += (g_vxx_0 - g_vxx_p) + (g_vxz_m - g_vxz_0) + (g_vxy_m - g_vxy_0);
+        vb1[t] += (g_vyy_0 - g_vyy_p) + (g_vyx_m - g_vyx_0) + (g_vyz_m - g_vyz_0);
+        vb2[t] += (g_vzz_0 - g_vzz_p) + (g_vzx_m - g_vzx_0) + (g_vzy_m - g_vzy_0);
 
         // λ, μ accumulators: forward e at p from v^n and ψv^n
-        const float *V0 = A.fn.v[0], *V1 = A.fn.v[1], *V2 = A.fn.v[2];
         const float v0 = V0[t], v1 = V1[t], v2 = V2[t];
         const float d_xx = v0 - V0[t - 1], d_yy = v1 - V1[t - sxz], d_zz = v2 - V2[t - nx];
         const float d_xz = V0[t + nx] - v0, d_xy = V0[t + sxz] - v0;
@@ -339,29 +385,28 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_stress(Geo g, AdjArgs A, int
         float fxx = d_xx, fyy = d_yy, fzz = d_zz, fxz = d_xz, fxy = d_xy;
         float fyx = d_yx, fyz = d_yz, fzx = d_zx, fzy = d_zy;
         if (sx0 >= 0) {
-            const long long pi = px_index(g, s, 0, y, z, sx0);
-            fxx += bx0 * A.fn.px[pi] + ax0 * d_xx;
-            fyx += bx0 * A.fn.px[pi + csx] + ax0 * d_yx;
-            fzx += bx0 * A.fn.px[pi + 2 * csx] + ax0 * d_zx;
+            const int pi = px_off(g, y, z, sx0);
+            fxx += bx0 * Fx[pi] + ax0 * d_xx;
+            fyx += bx0 * Fx[pi + csx] + ax0 * d_yx;
+            fzx += bx0 * Fx[pi + 2 * csx] + ax0 * d_zx;
         }
         if (sy0 >= 0) {
-            const long long pi = py_index(g, s, 0, sy0, z, x);
-            fxy += by0 * A.fn.py[pi] + ay0 * d_xy;
-            fyy += by0 * A.fn.py[pi + csy] + ay0 * d_yy;
-            fzy += by0 * A.fn.py[pi + 2 * csy] + ay0 * d_zy;
+            const int pi = py_off(g, sy0, z, x);
+            fxy += by0 * Fy[pi] + ay0 * d_xy;
+            fyy += by0 * Fy[pi + csy] + ay0 * d_yy;
+            fzy += by0 * Fy[pi + 2 * csy] + ay0 * d_zy;
         }
         if (sz0 >= 0) {
-            const long long pi = pz_index(g, s, 0, y, sz0, x);
-            fxz += bz0 * A.fn.pz[pi] + az0 * d_xz;
-            fyz += bz0 * A.fn.pz[pi + csz] + az0 * d_yz;
-            fzz += bz0 * A.fn.pz[pi + 2 * csz] + az0 * d_zz;
+            const int pi = pz_off(g, y, sz0, x);
+            fxz += bz0 * Fz[pi] + az0 * d_xz;
+            fyz += bz0 * Fz[pi + csz] + az0 * d_yz;
+            fzz += bz0 * Fz[pi + 2 * csz] + az0 * d_zz;
         }
-        float *acc = A.acc + (long long)s * 8 * g.ps + lc;
-        acc[0] += (txx + tyy + tzz) * (fxx + fyy + fzz);
-        acc[g.ps] += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
-        acc[2 * g.ps] += txy * (fxy + fyx);
-        acc[3 * g.ps] += txz * (fxz + fzx);
-        acc[4 * g.ps] += tyz * (fyz + fzy);
+        acc0[t] += (txx + tyy + tzz) * (fxx + fyy + fzz);
+        acc1[t] += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
+        acc2[t] += txy * (fxy + fyx);
+        acc3[t] += txz * (fxz + fzx);
+        acc4[t] += tyz * (fyz + fzy);
     }
 }
 

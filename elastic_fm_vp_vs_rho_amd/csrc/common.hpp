@@ -22,6 +22,8 @@ struct Geo {
     long long N;                       // nx*ny*nz
     long long sxz;                     // nx*nz (y stride)
     long long ps;                      // padded component stride N + EFM_PAD (acc, mbar, bbar)
+    int sxz32;                         // nx*nz as int (per-shot offsets are 32-bit)
+    int pxc, pyc, pzc;                 // per-component strides of the ψ slabs
 };
 
 struct StatePtrs {
@@ -49,6 +51,10 @@ inline Geo make_geo(const EfmDims &d) {
     g.sxz = (long long)d.nx * d.nz;
     g.N = g.sxz * d.ny;
     g.ps = g.N + EFM_PAD;
+    g.sxz32 = d.nx * d.nz;
+    g.pxc = d.ny * d.nz * g.Lx;
+    g.pyc = g.Ly * d.nz * d.nx;
+    g.pzc = d.ny * g.Lz * d.nx;
     return g;
 }
 
@@ -67,6 +73,14 @@ inline ShotPtrs make_shots(const EfmShots &s) {
     p.src = s.src; p.m6 = s.m6; p.wav = s.wavelet; p.nt = s.nt; p.rec = s.rec;
     return p;
 }
+
+// Per-shot ψ slab bases: comp c of shot s starts at base + c*pxc (int offsets below).
+__device__ __forceinline__ long long px_shot(const Geo &g, int s) { return (long long)s * 6 * g.pxc; }
+__device__ __forceinline__ long long py_shot(const Geo &g, int s) { return (long long)s * 6 * g.pyc; }
+__device__ __forceinline__ long long pz_shot(const Geo &g, int s) { return (long long)s * 6 * g.pzc; }
+__device__ __forceinline__ int px_off(const Geo &g, int y, int z, int sx) { return (y * g.nz + z) * g.Lx + sx; }
+__device__ __forceinline__ int py_off(const Geo &g, int sy, int z, int x) { return (sy * g.nz + z) * g.nx + x; }
+__device__ __forceinline__ int pz_off(const Geo &g, int y, int sz, int x) { return (y * g.Lz + sz) * g.nx + x; }
 
 // ---- slab addressing (ψ lives only in the CPML slabs) ----------------------
 // Returns the slab coordinate along an axis of length n with widths lo/hi, or -1.
@@ -110,18 +124,18 @@ struct Tile {
 
 // Compiled tile variants (bx, bz, ny, occ).  Whole 64-lane waves along x.
 #define EFM_TILE_LIST(X) \
+    X(64, 1, 1, 0)       \
+    X(64, 2, 1, 0)       \
     X(64, 4, 1, 0)       \
+    X(64, 8, 1, 0)       \
+    X(128, 1, 1, 0)      \
+    X(128, 2, 1, 0)      \
+    X(256, 1, 1, 0)      \
+    X(64, 2, 2, 0)       \
+    X(64, 4, 2, 0)       \
     X(64, 4, 4, 0)       \
-    X(64, 4, 8, 0)       \
-    X(64, 2, 8, 0)       \
-    X(64, 8, 4, 0)       \
-    X(64, 4, 16, 0)      \
-    X(128, 2, 4, 0)      \
-    X(64, 1, 16, 0)      \
-    X(64, 4, 1, 4)       \
-    X(64, 4, 4, 4)       \
-    X(64, 2, 8, 4)       \
-    X(64, 8, 2, 4)
+    X(128, 2, 2, 0)      \
+    X(64, 4, 1, 4)
 
 inline dim3 sweep_grid(const Geo &g, const Tile &t) {
     const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;

@@ -25,9 +25,11 @@ struct Med {
 // ---------------------------------------------------------------------------
 // stress sweep
 // ---------------------------------------------------------------------------
+// Addressing: per-shot base pointers are wave-uniform (scalar registers) and
+// every per-lane offset is a 32-bit int, so loads use the base+offset form.
 template <int BX, int BZ, int NYC, int OCC>
 __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh, StatePtrs in,
-                                                  StatePtrs out, int it, int src_mode) {
+                                                       StatePtrs out, int it, int src_mode) {
     const int x = blockIdx.x * BX + threadIdx.x;
     const int z = blockIdx.y * BZ + threadIdx.y;
     const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
@@ -37,8 +39,7 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh
     if (it >= sh.nt[s]) return;
     const int y0 = H + yb * NYC;
     const int y1 = min(y0 + NYC, g.ny - H);
-    const int nx = g.nx;
-    const long long sxz = g.sxz;
+    const int nx = g.nx, sxz = g.sxz32;
     const long long so = (long long)s * g.N;
     const float c = g.c;
 
@@ -53,15 +54,21 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh
     const bool src_col = (src_mode != 0) && (x == sp[0]) && (z == sp[2]);
     const int srcy = sp[1];
 
-    const float *__restrict__ vx = in.v[0];
-    const float *__restrict__ vy = in.v[1];
-    const float *__restrict__ vz = in.v[2];
+    const float *__restrict__ vx = in.v[0] + so;
+    const float *__restrict__ vy = in.v[1] + so;
+    const float *__restrict__ vz = in.v[2] + so;
     const float *__restrict__ mu = m.mu;
     const float *__restrict__ lam = m.lam;
+    const float *pxi = in.px + px_shot(g, s), *pyi = in.py + py_shot(g, s), *pzi = in.pz + pz_shot(g, s);
+    float *pxo = out.px + px_shot(g, s), *pyo = out.py + py_shot(g, s), *pzo = out.pz + pz_shot(g, s);
+    // τ in/out may alias (in-place update): no restrict on them
+    const float *ti0 = in.t[0] + so, *ti1 = in.t[1] + so, *ti2 = in.t[2] + so;
+    const float *ti3 = in.t[3] + so, *ti4 = in.t[4] + so, *ti5 = in.t[5] + so;
+    float *to0 = out.t[0] + so, *to1 = out.t[1] + so, *to2 = out.t[2] + so;
+    float *to3 = out.t[3] + so, *to4 = out.t[4] + so, *to5 = out.t[5] + so;
 
     for (int y = y0; y < y1; ++y) {
-        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
-        const long long t = so + lc;
+        const int t = x + z * nx + y * sxz;
         const float vxc = vx[t], vyc = vy[t], vzc = vz[t];
         const float dvx_dx = vxc - vx[t - 1];
         const float dvy_dy = vyc - vy[t - sxz];
@@ -77,51 +84,51 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh
         float exy = dvx_dy, eyy = dvy_dy, ezy = dvz_dy;
         float exz = dvx_dz, eyz = dvy_dz, ezz = dvz_dz;
         if (sxs >= 0) {
-            const long long pi = px_index(g, s, 0, y, z, sxs), cs = px_cstride(g);
-            const float p0 = bx * in.px[pi] + ax * dvx_dx;
-            const float p1 = bx * in.px[pi + cs] + ax * dvy_dx;
-            const float p2 = bx * in.px[pi + 2 * cs] + ax * dvz_dx;
-            out.px[pi] = p0; out.px[pi + cs] = p1; out.px[pi + 2 * cs] = p2;
+            const int pi = px_off(g, y, z, sxs), cs = g.pxc;
+            const float p0 = bx * pxi[pi] + ax * dvx_dx;
+            const float p1 = bx * pxi[pi + cs] + ax * dvy_dx;
+            const float p2 = bx * pxi[pi + 2 * cs] + ax * dvz_dx;
+            pxo[pi] = p0; pxo[pi + cs] = p1; pxo[pi + 2 * cs] = p2;
             exx += p0; eyx += p1; ezx += p2;
         }
         const int sys = slab_coord(y, g.ny, g.wyl, g.wyh);
         if (sys >= 0) {
             const float ay = sh.ay[s * g.ny + y], by = sh.by[s * g.ny + y];
-            const long long pi = py_index(g, s, 0, sys, z, x), cs = py_cstride(g);
-            const float p0 = by * in.py[pi] + ay * dvx_dy;
-            const float p1 = by * in.py[pi + cs] + ay * dvy_dy;
-            const float p2 = by * in.py[pi + 2 * cs] + ay * dvz_dy;
-            out.py[pi] = p0; out.py[pi + cs] = p1; out.py[pi + 2 * cs] = p2;
+            const int pi = py_off(g, sys, z, x), cs = g.pyc;
+            const float p0 = by * pyi[pi] + ay * dvx_dy;
+            const float p1 = by * pyi[pi + cs] + ay * dvy_dy;
+            const float p2 = by * pyi[pi + 2 * cs] + ay * dvz_dy;
+            pyo[pi] = p0; pyo[pi + cs] = p1; pyo[pi + 2 * cs] = p2;
             exy += p0; eyy += p1; ezy += p2;
         }
         if (szs >= 0) {
-            const long long pi = pz_index(g, s, 0, y, szs, x), cs = pz_cstride(g);
-            const float p0 = bz * in.pz[pi] + az * dvx_dz;
-            const float p1 = bz * in.pz[pi + cs] + az * dvy_dz;
-            const float p2 = bz * in.pz[pi + 2 * cs] + az * dvz_dz;
-            out.pz[pi] = p0; out.pz[pi + cs] = p1; out.pz[pi + 2 * cs] = p2;
+            const int pi = pz_off(g, y, szs, x), cs = g.pzc;
+            const float p0 = bz * pzi[pi] + az * dvx_dz;
+            const float p1 = bz * pzi[pi + cs] + az * dvy_dz;
+            const float p2 = bz * pzi[pi + 2 * cs] + az * dvz_dz;
+            pzo[pi] = p0; pzo[pi + cs] = p1; pzo[pi + 2 * cs] = p2;
             exz += p0; eyz += p1; ezz += p2;
         }
         // moduli: λ, μ and the three 4-point harmonic means of μ (M:80-89)
-        const float L = lam[lc];
-        const float r0 = frcp(mu[lc]);
-        const float ri = frcp(mu[lc + 1]);
-        const float rk = frcp(mu[lc + nx]);
-        const float rj = frcp(mu[lc + sxz]);
-        const float rik = frcp(mu[lc + 1 + nx]);
-        const float rjk = frcp(mu[lc + nx + sxz]);
-        const float rij = frcp(mu[lc + 1 + sxz]);
-        const float Mu = mu[lc];
+        const float L = lam[t];
+        const float Mu = mu[t];
+        const float r0 = frcp(Mu);
+        const float ri = frcp(mu[t + 1]);
+        const float rk = frcp(mu[t + nx]);
+        const float rj = frcp(mu[t + sxz]);
+        const float rik = frcp(mu[t + 1 + nx]);
+        const float rjk = frcp(mu[t + nx + sxz]);
+        const float rij = frcp(mu[t + 1 + sxz]);
         const float mxz = 4.f * frcp(r0 + ri + rk + rik);
         const float mxy = 4.f * frcp(r0 + ri + rj + rij);
         const float myz = 4.f * frcp(r0 + rj + rk + rjk);
         const float l2m = L + 2.f * Mu;
-        float txx = in.t[0][t] + c * (l2m * exx + L * (eyy + ezz));
-        float tyy = in.t[1][t] + c * (l2m * eyy + L * (exx + ezz));
-        float tzz = in.t[2][t] + c * (l2m * ezz + L * (exx + eyy));
-        float txy = in.t[3][t] + c * mxy * (exy + eyx);
-        float txz = in.t[4][t] + c * mxz * (exz + ezx);
-        float tyz = in.t[5][t] + c * myz * (eyz + ezy);
+        float txx = ti0[t] + c * (l2m * exx + L * (eyy + ezz));
+        float tyy = ti1[t] + c * (l2m * eyy + L * (exx + ezz));
+        float tzz = ti2[t] + c * (l2m * ezz + L * (exx + eyy));
+        float txy = ti3[t] + c * mxy * (exy + eyx);
+        float txz = ti4[t] + c * mxz * (exz + ezx);
+        float tyz = ti5[t] + c * myz * (eyz + ezy);
         if (src_col && y == srcy) {
             const int iw = src_mode > 0 ? it : sh.nt[s] - 1 - it;
             const float w0 = sh.wav[(long long)s * g.ntmax + iw];
@@ -130,8 +137,8 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh
             txx += m6[0] * w; tyy += m6[1] * w; tzz += m6[2] * w;
             txy += m6[3] * w; txz += m6[4] * w; tyz += m6[5] * w;
         }
-        out.t[0][t] = txx; out.t[1][t] = tyy; out.t[2][t] = tzz;
-        out.t[3][t] = txy; out.t[4][t] = txz; out.t[5][t] = tyz;
+        to0[t] = txx; to1[t] = tyy; to2[t] = tzz;
+        to3[t] = txy; to4[t] = txz; to5[t] = tyz;
     }
 }
 
@@ -140,7 +147,7 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh
 // ---------------------------------------------------------------------------
 template <int BX, int BZ, int NYC, int OCC>
 __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
-                                                    StatePtrs out, int it) {
+                                                         StatePtrs out, int it) {
     const int x = blockIdx.x * BX + threadIdx.x;
     const int z = blockIdx.y * BZ + threadIdx.y;
     const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
@@ -150,8 +157,7 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_velocity(Geo g, Med m, ShotPtrs 
     if (it >= sh.nt[s]) return;
     const int y0 = H + yb * NYC;
     const int y1 = min(y0 + NYC, g.ny - H);
-    const int nx = g.nx;
-    const long long sxz = g.sxz;
+    const int nx = g.nx, sxz = g.sxz32;
     const long long so = (long long)s * g.N;
     const float c = g.c;
 
@@ -162,17 +168,21 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_velocity(Geo g, Med m, ShotPtrs 
     if (szs >= 0) { az = sh.az[s * g.nz + z]; bz = sh.bz[s * g.nz + z]; }
 
     // τ is read from `out` (the stress sweep of this step wrote it there)
-    const float *__restrict__ txx = out.t[0];
-    const float *__restrict__ tyy = out.t[1];
-    const float *__restrict__ tzz = out.t[2];
-    const float *__restrict__ txy = out.t[3];
-    const float *__restrict__ txz = out.t[4];
-    const float *__restrict__ tyz = out.t[5];
+    const float *__restrict__ txx = out.t[0] + so;
+    const float *__restrict__ tyy = out.t[1] + so;
+    const float *__restrict__ tzz = out.t[2] + so;
+    const float *__restrict__ txy = out.t[3] + so;
+    const float *__restrict__ txz = out.t[4] + so;
+    const float *__restrict__ tyz = out.t[5] + so;
     const float *__restrict__ rho = m.rho;
+    const float *pxi = in.px + px_shot(g, s), *pyi = in.py + py_shot(g, s), *pzi = in.pz + pz_shot(g, s);
+    float *pxo = out.px + px_shot(g, s), *pyo = out.py + py_shot(g, s), *pzo = out.pz + pz_shot(g, s);
+    // v in/out may alias (in-place update)
+    const float *vi0 = in.v[0] + so, *vi1 = in.v[1] + so, *vi2 = in.v[2] + so;
+    float *vo0 = out.v[0] + so, *vo1 = out.v[1] + so, *vo2 = out.v[2] + so;
 
     for (int y = y0; y < y1; ++y) {
-        const long long lc = (long long)x + (long long)z * nx + (long long)y * sxz;
-        const long long t = so + lc;
+        const int t = x + z * nx + y * sxz;
         const float dtxx_dx = txx[t + 1] - txx[t];
         const float dtzz_dz = tzz[t + nx] - tzz[t];
         const float dtyy_dy = tyy[t + sxz] - tyy[t];
@@ -187,39 +197,39 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_velocity(Geo g, Med m, ShotPtrs 
         float fxy = dtxy_dy, fyy = dtyy_dy, fzy = dtyz_dy;
         float fxz = dtxz_dz, fyz = dtyz_dz, fzz = dtzz_dz;
         if (sxs >= 0) {
-            const long long pi = px_index(g, s, 3, y, z, sxs), cs = px_cstride(g);
-            const float p0 = bx * in.px[pi] + ax * dtxx_dx;
-            const float p1 = bx * in.px[pi + cs] + ax * dtxy_dx;
-            const float p2 = bx * in.px[pi + 2 * cs] + ax * dtxz_dx;
-            out.px[pi] = p0; out.px[pi + cs] = p1; out.px[pi + 2 * cs] = p2;
+            const int cs = g.pxc, pi = px_off(g, y, z, sxs) + 3 * cs;
+            const float p0 = bx * pxi[pi] + ax * dtxx_dx;
+            const float p1 = bx * pxi[pi + cs] + ax * dtxy_dx;
+            const float p2 = bx * pxi[pi + 2 * cs] + ax * dtxz_dx;
+            pxo[pi] = p0; pxo[pi + cs] = p1; pxo[pi + 2 * cs] = p2;
             fxx += p0; fyx += p1; fzx += p2;
         }
         const int sys = slab_coord(y, g.ny, g.wyl, g.wyh);
         if (sys >= 0) {
             const float ay = sh.ay[s * g.ny + y], by = sh.by[s * g.ny + y];
-            const long long pi = py_index(g, s, 3, sys, z, x), cs = py_cstride(g);
-            const float p0 = by * in.py[pi] + ay * dtxy_dy;
-            const float p1 = by * in.py[pi + cs] + ay * dtyy_dy;
-            const float p2 = by * in.py[pi + 2 * cs] + ay * dtyz_dy;
-            out.py[pi] = p0; out.py[pi + cs] = p1; out.py[pi + 2 * cs] = p2;
+            const int cs = g.pyc, pi = py_off(g, sys, z, x) + 3 * cs;
+            const float p0 = by * pyi[pi] + ay * dtxy_dy;
+            const float p1 = by * pyi[pi + cs] + ay * dtyy_dy;
+            const float p2 = by * pyi[pi + 2 * cs] + ay * dtyz_dy;
+            pyo[pi] = p0; pyo[pi + cs] = p1; pyo[pi + 2 * cs] = p2;
             fxy += p0; fyy += p1; fzy += p2;
         }
         if (szs >= 0) {
-            const long long pi = pz_index(g, s, 3, y, szs, x), cs = pz_cstride(g);
-            const float p0 = bz * in.pz[pi] + az * dtxz_dz;
-            const float p1 = bz * in.pz[pi + cs] + az * dtyz_dz;
-            const float p2 = bz * in.pz[pi + 2 * cs] + az * dtzz_dz;
-            out.pz[pi] = p0; out.pz[pi + cs] = p1; out.pz[pi + 2 * cs] = p2;
+            const int cs = g.pzc, pi = pz_off(g, y, szs, x) + 3 * cs;
+            const float p0 = bz * pzi[pi] + az * dtxz_dz;
+            const float p1 = bz * pzi[pi + cs] + az * dtyz_dz;
+            const float p2 = bz * pzi[pi + 2 * cs] + az * dtzz_dz;
+            pzo[pi] = p0; pzo[pi + cs] = p1; pzo[pi + 2 * cs] = p2;
             fxz += p0; fyz += p1; fzz += p2;
         }
         // averaged buoyancy b̄ = 2bb'/(b+b') = 2/(ρ+ρ')  (M:153-159)
-        const float r = rho[lc];
-        const float bbx = 2.f * frcp(r + rho[lc + 1]);
-        const float bby = 2.f * frcp(r + rho[lc + sxz]);
-        const float bbz = 2.f * frcp(r + rho[lc + nx]);
-        out.v[0][t] = in.v[0][t] + c * bbx * (fxx + fxy + fxz);
-        out.v[1][t] = in.v[1][t] + c * bby * (fyx + fyy + fyz);
-        out.v[2][t] = in.v[2][t] + c * bbz * (fzx + fzy + fzz);
+        const float r = rho[t];
+        const float bbx = 2.f * frcp(r + rho[t + 1]);
+        const float bby = 2.f * frcp(r + rho[t + sxz]);
+        const float bbz = 2.f * frcp(r + rho[t + nx]);
+        vo0[t] = vi0[t] + c * bbx * (fxx + fxy + fxz);
+        vo1[t] = vi1[t] + c * bby * (fyx + fyy + fyz);
+        vo2[t] = vi2[t] + c * bbz * (fzx + fzy + fzz);
     }
 }
 
