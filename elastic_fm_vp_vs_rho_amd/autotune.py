@@ -2,7 +2,7 @@
 
 For each sweep kernel (0 stress, 1 velocity, 2 adjoint velocity, 3 adjoint
 stress) every compiled tile variant (bx × bz threads, ny cells per thread along
-y, occupancy floor) is first run once on a live state and its output compared
+y, block-to-XCD mapping) is first run once on a live state and its output compared
 with the output of the reference tile; a candidate that does not reproduce it
 (relative max-norm difference above 1e-6 on any field) is rejected before it is
 timed.  Survivors are timed with HIP events on the library's stream; the
@@ -21,7 +21,7 @@ from . import _capi
 from .propagator import Wavefield, _ptr, _stream, padded_components
 
 KERNELS = ("stress", "velocity", "adj_velocity", "adj_stress")
-REF_TILE = (64, 4, 4, 0)
+REF_TILE = (64, 4, 1, 0)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "efm_tiles.json")
 
 

@@ -37,13 +37,10 @@ struct AdjArgs {
 // ---------------------------------------------------------------------------
 // R2: transpose of the velocity sweep
 // ---------------------------------------------------------------------------
-template <int BX, int BZ, int NYC, int OCC>
-__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_velocity(Geo g, AdjArgs A, int n) {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int z = blockIdx.y * BZ + threadIdx.y;
-    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
-    const int s = blockIdx.z / nyc;
-    const int yb = blockIdx.z - s * nyc;
+template <int BX, int BZ, int NYC, int MAP>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, int n) {
+    int x, z, s, yb;
+    if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
     if (n >= A.sh.nt[s]) return;
     const int y0 = H + yb * NYC;
@@ -217,13 +214,10 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_velocity(Geo g, AdjArgs A, i
 // ---------------------------------------------------------------------------
 // R4: transpose of the stress sweep
 // ---------------------------------------------------------------------------
-template <int BX, int BZ, int NYC, int OCC>
-__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_adj_stress(Geo g, AdjArgs A, int n) {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int z = blockIdx.y * BZ + threadIdx.y;
-    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
-    const int s = blockIdx.z / nyc;
-    const int yb = blockIdx.z - s * nyc;
+template <int BX, int BZ, int NYC, int MAP>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_stress(Geo g, AdjArgs A, int n) {
+    int x, z, s, yb;
+    if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
     if (n >= A.sh.nt[s]) return;
     const int y0 = H + yb * NYC;

@@ -26,7 +26,7 @@ int check_launch(const char *what) {
 
 // Tiles: 0 stress, 1 velocity, 2 adjoint velocity, 3 adjoint stress.
 static std::mutex g_tile_mu;
-static Tile g_tiles[4] = {{64, 4, 4, 0}, {64, 4, 4, 0}, {64, 4, 4, 0}, {64, 4, 4, 0}};
+static Tile g_tiles[4] = {{64, 4, 1, 0}, {64, 4, 1, 0}, {64, 4, 1, 0}, {64, 4, 1, 0}};
 #define EFM_TILE_ENTRY(a, b, c, o) {a, b, c, o},
 static const Tile kTiles[] = {EFM_TILE_LIST(EFM_TILE_ENTRY)};
 #undef EFM_TILE_ENTRY
@@ -39,7 +39,7 @@ Tile get_tile(int kernel) {
 int set_tile(int kernel, Tile t) {
     if (kernel < 0 || kernel > 3) return fail(EFM_E_ARG, "tile kernel id out of range");
     for (const Tile &k : kTiles)
-        if (k.bx == t.bx && k.bz == t.bz && k.ny == t.ny && k.occ == t.occ) {
+        if (k.bx == t.bx && k.bz == t.bz && k.ny == t.ny && k.map == t.map) {
             std::lock_guard<std::mutex> lk(g_tile_mu);
             g_tiles[kernel] = t;
             return EFM_OK;
@@ -88,15 +88,15 @@ extern "C" int efm_last_error(char *buf, int len) {
 
 extern "C" int efm_component_pad(void) { return EFM_PAD; }
 
-extern "C" int efm_set_tile(int kernel, int bx, int bz, int ny, int occ) {
-    return set_tile(kernel, Tile{bx, bz, ny, occ});
+extern "C" int efm_set_tile(int kernel, int bx, int bz, int ny, int map) {
+    return set_tile(kernel, Tile{bx, bz, ny, map});
 }
 
-extern "C" int efm_get_tile(int kernel, int *bx, int *bz, int *ny, int *occ) {
-    if (kernel < 0 || kernel > 3 || !bx || !bz || !ny || !occ)
+extern "C" int efm_get_tile(int kernel, int *bx, int *bz, int *ny, int *map) {
+    if (kernel < 0 || kernel > 3 || !bx || !bz || !ny || !map)
         return fail(EFM_E_ARG, "efm_get_tile: bad argument");
     const Tile t = get_tile(kernel);
-    *bx = t.bx; *bz = t.bz; *ny = t.ny; *occ = t.occ;
+    *bx = t.bx; *bz = t.bz; *ny = t.ny; *map = t.map;
     return EFM_OK;
 }
 
@@ -105,7 +105,7 @@ extern "C" int efm_tile_variants(int *out, int max) {
     if (out)
         for (int i = 0; i < n && i < max; ++i) {
             out[4 * i] = kTiles[i].bx; out[4 * i + 1] = kTiles[i].bz;
-            out[4 * i + 2] = kTiles[i].ny; out[4 * i + 3] = kTiles[i].occ;
+            out[4 * i + 2] = kTiles[i].ny; out[4 * i + 3] = kTiles[i].map;
         }
     return n;
 }

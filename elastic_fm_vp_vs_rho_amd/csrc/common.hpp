@@ -119,38 +119,84 @@ int check_launch(const char *what);
 // ---- tile configuration chosen by the autotuner ------------------------------
 struct Tile {
     int bx, bz, ny;  // threads in x, threads in z, cells per thread along y
-    int occ;         // minimum waves per SIMD the compiler must allow (0: its own choice)
+    int map;         // block mapping: 0 = linear 3-D grid; k > 0 = XCD-aware over k XCDs
 };
 
-// Compiled tile variants (bx, bz, ny, occ).  Whole 64-lane waves along x.
+// Compiled tile variants (bx, bz, ny, map).  Whole 64-lane waves along x.
 #define EFM_TILE_LIST(X) \
     X(64, 1, 1, 0)       \
     X(64, 2, 1, 0)       \
     X(64, 4, 1, 0)       \
-    X(64, 8, 1, 0)       \
     X(128, 1, 1, 0)      \
     X(128, 2, 1, 0)      \
-    X(256, 1, 1, 0)      \
-    X(64, 2, 2, 0)       \
     X(64, 4, 2, 0)       \
-    X(64, 4, 4, 0)       \
-    X(128, 2, 2, 0)      \
-    X(64, 4, 1, 4)
+    X(64, 1, 1, 8)       \
+    X(64, 2, 1, 8)       \
+    X(64, 4, 1, 8)       \
+    X(64, 8, 1, 8)       \
+    X(128, 1, 1, 8)      \
+    X(128, 2, 1, 8)      \
+    X(256, 1, 1, 8)      \
+    X(64, 4, 2, 8)
+
+// Number of tiles (x-blocks × z-blocks × y-chunks × shots) of a sweep.
+inline long long sweep_tiles(const Geo &g, int bx, int bz, int ny) {
+    const long long nbx = (g.nx + bx - 1) / bx, nbz = (g.nz + bz - 1) / bz;
+    const long long nyc = (g.ny - 2 * H + ny - 1) / ny;
+    return nbx * nbz * nyc * g.S;
+}
 
 inline dim3 sweep_grid(const Geo &g, const Tile &t) {
+    if (t.map > 0) {
+        const long long B = sweep_tiles(g, t.bx, t.bz, t.ny);
+        return dim3((unsigned)(t.map * ((B + t.map - 1) / t.map)));
+    }
     const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
     return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
 }
 
-// Launch KER<bx,bz,ny,occ> for the selected tile; sets `launched`.
-#define EFM_LAUNCH_IF(BX_, BZ_, NY_, OCC_, KER, ...)                                           \
-    if (!launched && t.bx == BX_ && t.bz == BZ_ && t.ny == NY_ && t.occ == OCC_) {              \
-        hipLaunchKernelGGL((KER<BX_, BZ_, NY_, OCC_>), sweep_grid(g, t), dim3(BX_, BZ_), 0, st, \
+// Tile coordinates of the calling block.  MAP = 0: the 3-D grid as launched.
+// MAP = k: workgroups are dealt to the k XCDs round-robin (w mod k), each with
+// its own L2, so block w is given tile k_w·per + w/k: every XCD walks a
+// contiguous run of tiles (x fastest, then z, then y), and the ±z rows and ±y
+// planes a tile reads were fetched into the same L2 by its neighbours.
+// Returns false for the padding blocks past the last tile.
+template <int BX, int BZ, int NYC, int MAP>
+__device__ __forceinline__ bool tile_coords(const Geo &g, int &x, int &z, int &s, int &yb) {
+    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
+    if (MAP == 0) {
+        x = blockIdx.x * BX + threadIdx.x;
+        z = blockIdx.y * BZ + threadIdx.y;
+        s = blockIdx.z / nyc;
+        yb = blockIdx.z - s * nyc;
+        return true;
+    }
+    const int nbx = (g.nx + BX - 1) / BX, nbz = (g.nz + BZ - 1) / BZ;
+    const int B = nbx * nbz * nyc * g.S;
+    const int per = (B + MAP - 1) / MAP;
+    const int w = blockIdx.x;
+    const int gi = (w % MAP) * per + w / MAP;
+    if (gi >= B) return false;
+    const int xb = gi % nbx;
+    int r = gi / nbx;
+    const int zb = r % nbz;
+    r /= nbz;
+    yb = r % nyc;
+    s = r / nyc;
+    x = xb * BX + threadIdx.x;
+    z = zb * BZ + threadIdx.y;
+    return true;
+}
+
+// Launch KER<bx,bz,ny,map> for the selected tile; sets `launched`.
+#define EFM_LAUNCH_IF(BX_, BZ_, NY_, MAP_, KER, ...)                                           \
+    if (!launched && t.bx == BX_ && t.bz == BZ_ && t.ny == NY_ && t.map == MAP_) {              \
+        hipLaunchKernelGGL((KER<BX_, BZ_, NY_, MAP_>), sweep_grid(g, t), dim3(BX_, BZ_), 0, st, \
                            __VA_ARGS__);                                                        \
         launched = true;                                                                       \
     }
-#define EFM_KERNEL_ATTRS(BX, BZ, OCC) \
-    __launch_bounds__(BX *BZ) __attribute__((amdgpu_waves_per_eu((OCC) > 0 ? (OCC) : 1)))
+#define EFM_KERNEL_ATTRS(BX, BZ, MAP) __launch_bounds__(BX *BZ)
+
 Tile get_tile(int kernel);  // 0 stress, 1 velocity, 2 adjoint-velocity, 3 adjoint-stress
 int set_tile(int kernel, Tile t);
 

@@ -27,14 +27,11 @@ struct Med {
 // ---------------------------------------------------------------------------
 // Addressing: per-shot base pointers are wave-uniform (scalar registers) and
 // every per-lane offset is a 32-bit int, so loads use the base+offset form.
-template <int BX, int BZ, int NYC, int OCC>
-__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh, StatePtrs in,
+template <int BX, int BZ, int NYC, int MAP>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_stress(Geo g, Med m, ShotPtrs sh, StatePtrs in,
                                                        StatePtrs out, int it, int src_mode) {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int z = blockIdx.y * BZ + threadIdx.y;
-    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
-    const int s = blockIdx.z / nyc;
-    const int yb = blockIdx.z - s * nyc;
+    int x, z, s, yb;
+    if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
     if (it >= sh.nt[s]) return;
     const int y0 = H + yb * NYC;
@@ -145,14 +142,11 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_stress(Geo g, Med m, ShotPtrs sh
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
-template <int BX, int BZ, int NYC, int OCC>
-__global__ EFM_KERNEL_ATTRS(BX, BZ, OCC) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
+template <int BX, int BZ, int NYC, int MAP>
+__global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
                                                          StatePtrs out, int it) {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int z = blockIdx.y * BZ + threadIdx.y;
-    const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
-    const int s = blockIdx.z / nyc;
-    const int yb = blockIdx.z - s * nyc;
+    int x, z, s, yb;
+    if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
     if (it >= sh.nt[s]) return;
     const int y0 = H + yb * NYC;

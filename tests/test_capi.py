@@ -49,6 +49,7 @@ def test_version_and_errors(L):
     buf = (ctypes.c_int * (4 * n))()
     assert L.efm_tile_variants(buf, n) == n
     assert all(buf[4 * i] % 64 == 0 for i in range(n))   # whole 64-lane waves along x
+    assert {buf[4 * i + 3] for i in range(n)} == {0, 8}   # linear and XCD-aware mappings
     assert L.efm_component_pad() == 1088
 
 
