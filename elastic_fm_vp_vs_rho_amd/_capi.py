@@ -73,7 +73,7 @@ _SIGS = {
     "efm_image_mt_ref": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), c_void_p, c_int,
                           c_void_p], c_int),
     "efm_backward_mt_ref": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), c_void_p, c_void_p,
-                             c_void_p], c_int),
+                             c_int, c_void_p], c_int),
     "efm_backward_remake_ref": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), c_void_p,
                                  c_void_p, c_int, P(EfmState), c_void_p, c_float, c_float, c_float,
                                  c_float, c_void_p, c_void_p, c_void_p, c_void_p], c_int),

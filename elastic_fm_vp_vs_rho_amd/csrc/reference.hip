@@ -163,17 +163,18 @@ extern "C" int efm_image_mt_ref(const EfmDims *d, const EfmMedium *m, const EfmS
 }
 
 extern "C" int efm_backward_mt_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
-                                   const EfmState *adj, const float *res, double *gm6,
+                                   const EfmState *adj, const float *res, double *gm6, int nsteps,
                                    void *stream) {
     int rc = check_common(d, m, sh);
     if (rc) return rc;
     if (!adj || !res || !gm6) return fail(EFM_E_ARG, "efm_backward_mt_ref: null argument");
+    if (nsteps < 0 || nsteps > d->ntmax) return fail(EFM_E_ARG, "efm_backward_mt_ref: nsteps out of range");
     Geo g = make_geo(*d);
     if (g.c < 0) g.c = -g.c;  // the adjoint runs the forward operator with +dt (M:2066-2068)
     hipStream_t st = (hipStream_t)stream;
     const ShotPtrs shp = make_shots(*sh);
     const StatePtrs a = make_state(*adj);
-    for (int it = 0; it < d->ntmax; ++it) {
+    for (int it = 0; it < nsteps; ++it) {
         if ((rc = launch_stress_src(g, *m, shp, a, a, it, 0, st))) return rc;
         if ((rc = launch_velocity(g, *m, shp, a, a, it, st))) return rc;
         if (g.nrec > 0) {

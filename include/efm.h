@@ -156,10 +156,12 @@ int efm_inject_residual(const EfmDims *d, const EfmShots *sh, const EfmState *st
 /* kernel_Gradients_Mij (M:337-372), accumulated: gm6[s] += w[nt-it-1]·C(τ†(src)). */
 int efm_image_mt_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
                      const EfmState *adj, double *gm6, int it, void *stream);
-/* backwardFields (M:2056-2082, mode 0): nt adjoint steps with the forward
- * operator (+dt), residual injection and MT imaging; adj must be zero. */
+/* backwardFields (M:2056-2082, mode 0): nsteps adjoint steps (nt there; nt-1
+ * inside backwardFields_FowardFieldsRemake) with the forward operator (+dt),
+ * residual injection and MT imaging; adj must be zero on entry. */
 int efm_backward_mt_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
-                        const EfmState *adj, const float *res, double *gm6, void *stream);
+                        const EfmState *adj, const float *res, double *gm6, int nsteps,
+                        void *stream);
 /* backwardFields_FowardFieldsRemake (M:1929-2054, mode 1), nt-1 iterations.
  * fwd is the final forward state (consumed), vpres is a 3-volume scratch
  * (S, 3, ny, nz, nx), borders from efm_forward, adj zero on entry.  Imaging
