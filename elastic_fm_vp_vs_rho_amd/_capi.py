@@ -61,6 +61,8 @@ _SIGS = {
     "efm_record": ([P(EfmDims), P(EfmShots), P(EfmState), c_void_p, c_int, c_void_p], c_int),
     "efm_forward": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), c_void_p, c_void_p, c_int,
                      c_int, c_int, c_void_p], c_int),
+    "efm_forward_graph": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), c_void_p, c_int, c_int,
+                           c_int, c_void_p, c_void_p], c_int),
     "efm_forward_states": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(P(EfmState)), c_void_p, c_int,
                             c_int, c_void_p], c_int),
     "efm_derived_medium": ([P(EfmDims), P(EfmMedium), c_void_p, c_void_p, c_void_p], c_int),

@@ -158,6 +158,70 @@ extern "C" int efm_forward(const EfmDims *d, const EfmMedium *m, const EfmShots 
     return EFM_OK;
 }
 
+namespace efm {
+__global__ void k_counter_set(int *c, int v) {
+    if (threadIdx.x == 0) *c = v;
+}
+__global__ void k_counter_add(int *c, int v) {
+    if (threadIdx.x == 0) *c += v;
+}
+}  // namespace efm
+
+extern "C" int efm_forward_graph(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                                 const EfmState *st, float *gathers, int it0, int nsteps, int chunk,
+                                 int *counter, void *stream) {
+    int rc = validate(d, m, sh);
+    if (rc || (rc = validate_state(st))) return rc;
+    if (it0 < 0 || nsteps < 0 || it0 + nsteps > d->ntmax)
+        return fail(EFM_E_ARG, "efm_forward_graph: step range outside [0, ntmax)");
+    if (chunk < 1 || !counter) return fail(EFM_E_ARG, "efm_forward_graph: chunk >= 1 and a counter are required");
+    hipStream_t hs = (hipStream_t)stream;
+    if (!hs) return fail(EFM_E_ARG, "efm_forward_graph: graph capture needs a non-default stream");
+    const ShotPtrs shp = make_shots(*sh);
+    const StatePtrs s = make_state(*st);
+    const int nchunks = nsteps / chunk;
+    if (nchunks > 0) {
+        hipLaunchKernelGGL(k_counter_set, dim3(1), dim3(64), 0, hs, counter, it0);
+        if ((rc = check_launch("counter set"))) return rc;
+        Geo g = make_geo(*d);
+        g.itp = counter;
+        if (hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail(EFM_E_HIP, "efm_forward_graph: hipStreamBeginCapture failed");
+        int crc = EFM_OK;
+        for (int k = 0; k < chunk && crc == EFM_OK; ++k) {
+            if ((crc = launch_stress_src(g, *m, shp, s, s, k, 1, hs))) break;
+            if ((crc = launch_velocity(g, *m, shp, s, s, k, hs))) break;
+            if (gathers) crc = launch_record(g, shp, s, gathers, k, hs);
+        }
+        if (crc == EFM_OK) {
+            hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, hs, counter, chunk);
+            crc = check_launch("counter add");
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(hs, &graph);
+        if (crc) { if (graph) (void)hipGraphDestroy(graph); return crc; }
+        if (ec != hipSuccess || !graph) return fail(EFM_E_HIP, "efm_forward_graph: hipStreamEndCapture failed");
+        hipGraphExec_t exec = nullptr;
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(EFM_E_HIP, "efm_forward_graph: hipGraphInstantiate failed");
+        }
+        hipError_t el = hipSuccess;
+        for (int c = 0; c < nchunks && el == hipSuccess; ++c) el = hipGraphLaunch(exec, hs);
+        const hipError_t es = hipStreamSynchronize(hs);   // the exec is released below
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (el != hipSuccess || es != hipSuccess) return fail(EFM_E_HIP, "efm_forward_graph: graph launch failed");
+    }
+    const Geo g = make_geo(*d);
+    for (int it = it0 + nchunks * chunk; it < it0 + nsteps; ++it) {
+        if ((rc = launch_stress_src(g, *m, shp, s, s, it, 1, hs))) return rc;
+        if ((rc = launch_velocity(g, *m, shp, s, s, it, hs))) return rc;
+        if (gathers && (rc = launch_record(g, shp, s, gathers, it, hs))) return rc;
+    }
+    return EFM_OK;
+}
+
 extern "C" int efm_forward_states(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
                                   const EfmState *const *states, float *gathers, int it0,
                                   int nsteps, void *stream) {

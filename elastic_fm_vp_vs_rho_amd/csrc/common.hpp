@@ -24,6 +24,7 @@ struct Geo {
     long long ps;                      // padded component stride N + EFM_PAD (acc, mbar, bbar)
     int sxz32;                         // nx*nz as int (per-shot offsets are 32-bit)
     int pxc, pyc, pzc;                 // per-component strides of the ψ slabs
+    const int *itp;                    // device time-step counter added to `it` (graph replay), or null
 };
 
 struct StatePtrs {
@@ -55,6 +56,7 @@ inline Geo make_geo(const EfmDims &d) {
     g.pxc = d.ny * d.nz * g.Lx;
     g.pyc = g.Ly * d.nz * d.nx;
     g.pzc = d.ny * g.Lz * d.nx;
+    g.itp = nullptr;
     return g;
 }
 

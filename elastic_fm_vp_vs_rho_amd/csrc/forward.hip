@@ -30,6 +30,7 @@ struct Med {
 template <int BX, int BZ, int NYC, int MAP>
 __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_stress(Geo g, Med m, ShotPtrs sh, StatePtrs in,
                                                        StatePtrs out, int it, int src_mode) {
+    if (g.itp) it += *g.itp;
     int x, z, s, yb;
     if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
@@ -145,6 +146,7 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_stress(Geo g, Med m, ShotPtrs sh
 template <int BX, int BZ, int NYC, int MAP>
 __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
                                                          StatePtrs out, int it) {
+    if (g.itp) it += *g.itp;
     int x, z, s, yb;
     if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
@@ -231,6 +233,7 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_velocity(Geo g, Med m, ShotPtrs 
 // receivers
 // ---------------------------------------------------------------------------
 __global__ void k_record(Geo g, ShotPtrs sh, StatePtrs st, float *gathers, int it) {
+    if (g.itp) it += *g.itp;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
     if (r >= g.nrec || it >= sh.nt[s]) return;

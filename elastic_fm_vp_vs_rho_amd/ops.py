@@ -1,0 +1,45 @@
+"""Seismograms as a differentiable torch function of (Vp, Vs, ρ, M).
+
+``elastic_modeling(vp, vs, rho, m6, grid, shots)`` returns the gathers
+(S, 3, ntmax, nrec) of a shot batch.  Its backward is the exact discrete
+adjoint of the HIP propagator (adjoint.hip): the forward keeps √nt checkpoints,
+the backward rebuilds each segment and runs the transposed sweeps, which
+cross-correlate forward and adjoint wavefields into ∂/∂λ, ∂/∂μ, ∂/∂ρ and
+∂/∂M; efm_model_lame_adjoint maps them to Vp, Vs, ρ.  No torch arithmetic is
+involved on either pass.
+"""
+from __future__ import annotations
+
+import torch
+
+from .propagator import Grid, Propagator, ShotBatch, lame, lame_adjoint
+
+
+class ElasticModeling(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vp, vs, rho, m6, grid: Grid, shots: ShotBatch, K):
+        for t in (vp, vs, rho):
+            if t.dtype != torch.float32 or not t.is_cuda:
+                raise TypeError("vp, vs, rho must be float32 device tensors")
+        vp, vs, rho = vp.contiguous(), vs.contiguous(), rho.contiguous()
+        shots.m6.copy_(m6.detach().reshape(shots.S, 6).to(torch.float32))
+        lam, mu = lame(vp, vs, rho)
+        prop = Propagator(grid, lam, mu, rho, shots)
+        g, snaps, K = prop.forward_checkpointed(K)
+        ctx.prop, ctx.snaps, ctx.K = prop, snaps, K
+        ctx.save_for_backward(vp, vs, rho)
+        ctx.m6_dtype = m6.dtype
+        return g
+
+    @staticmethod
+    def backward(ctx, grad_g):
+        vp, vs, rho = ctx.saved_tensors
+        glam, gmu, grho, gm6 = ctx.prop.gradient_from_snapshots(ctx.snaps, ctx.K, grad_g)
+        ctx.snaps = None
+        gvp, gvs, grho_total = lame_adjoint(vp, vs, rho, glam, gmu, grho)
+        return gvp, gvs, grho_total, gm6.to(ctx.m6_dtype), None, None, None
+
+
+def elastic_modeling(vp, vs, rho, m6, grid: Grid, shots: ShotBatch, K=None):
+    """Differentiable gathers of ``shots`` in the medium (vp, vs, rho)."""
+    return ElasticModeling.apply(vp, vs, rho, m6, grid, shots, K)

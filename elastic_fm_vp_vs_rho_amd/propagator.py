@@ -218,12 +218,28 @@ class Propagator:
         return torch.zeros((s.S, 3, s.ntmax, s.nrec), dtype=torch.float32, device=self.lam.device)
 
     # -- forward ------------------------------------------------------------
-    def forward(self, state=None, gathers=None, it0=0, nsteps=None, borders=None, W=0):
-        """Advance ``state`` (zeroed if None) through steps it0 .. it0+nsteps-1."""
+    def forward(self, state=None, gathers=None, it0=0, nsteps=None, borders=None, W=0,
+                graph_chunk=None):
+        """Advance ``state`` (zeroed if None) through steps it0 .. it0+nsteps-1.
+        ``graph_chunk`` = K replays one captured hipGraph of K steps instead of
+        launching every kernel (no border save in that mode)."""
         st = self.new_wavefield() if state is None else state
         g = self.new_gathers() if gathers is None else gathers
         n = self.shots.ntmax - it0 if nsteps is None else nsteps
         d, m, sh = self.dims(), self.medium(), self.shots.struct()
+        if graph_chunk and borders is None:
+            if getattr(self, "_gstream", None) is None:
+                self._gstream = torch.cuda.Stream(device=self.lam.device)
+                self._counter = torch.zeros(1, dtype=torch.int32, device=self.lam.device)
+            cur = torch.cuda.current_stream(self.lam.device)
+            self._gstream.wait_stream(cur)
+            with torch.cuda.stream(self._gstream):
+                check(self.lib.efm_forward_graph(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
+                                                 ctypes.byref(st.struct()), _ptr(g), int(it0), int(n),
+                                                 int(graph_chunk), _ptr(self._counter), _stream()),
+                      "efm_forward_graph")
+            cur.wait_stream(self._gstream)
+            return g, st
         check(self.lib.efm_forward(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
                                    ctypes.byref(st.struct()), _ptr(g), _ptr(borders), int(W),
                                    int(it0), int(n), _stream()), "efm_forward")

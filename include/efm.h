@@ -118,6 +118,15 @@ int efm_record(const EfmDims *d, const EfmShots *sh, const EfmState *st,
 int efm_forward(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
                 const EfmState *st, float *gathers, float *borders, int W,
                 int it0, int nsteps, void *stream);
+/* efm_forward without borders, with the time loop captured once as a hipGraph
+ * of `chunk` steps and replayed (no reference counterpart: the reference
+ * issues 9+ launches from Python per step).  The kernels read the step index
+ * from `counter`, a caller-provided device int, which the graph advances by
+ * `chunk` per replay; steps beyond the last whole chunk run eagerly.  Needs a
+ * non-default stream; returns after the work has completed. */
+int efm_forward_graph(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                      const EfmState *st, float *gathers, int it0, int nsteps, int chunk,
+                      int *counter, void *stream);
 /* Out-of-place forward for steps it0 .. it0+nsteps-1 (no reference
  * counterpart; used to rebuild a checkpoint segment): states[k] holds the
  * pre-step state S^(it0+k), k = 0 .. nsteps; step it0+k reads states[k] and
