@@ -125,9 +125,17 @@ def test_fwi_mij_nan_path_reverts_and_halves(gpu_lib):
 @pytest.mark.parametrize("gradient", ["reference", "exact"])
 def test_fwi_vp_vs_rho_runs_and_descends(gpu_lib, gradient):
     e3, mdl, wav, cpml, geo = _setup(nq=2, mode=1, n=(30, 28, 26), W=4, nt=120, seed=9)
+    # the model update re-derives ρ from Vp (Nafe-Drake), so start from such a model
+    # as the reference run does (cn.compute_Rho(), M:3072-3073)
+    mdl.compute_Rho()
+    mdl.Rho = np.asarray(mdl.Rho, np.float32)
+    mdl.compute_Lambda_Mu()
+    e3.setModel(mdl)
     tm = [D.StressTensor(1.0, 0.2, 1.0, 0.0, 0.3, 0.0)] * 2
     true = D.Model(mdl.nz, mdl.ny, mdl.nx, mdl.dz, mdl.dy, mdl.dx)
-    true.Vp, true.Vs, true.Rho = mdl.Vp * 1.03, mdl.Vs * 1.03, mdl.Rho.copy()
+    true.Vp, true.Vs = mdl.Vp * 1.03, mdl.Vs * 1.03
+    true.compute_Rho()
+    true.Rho = np.asarray(true.Rho, np.float32)
     true.compute_Lambda_Mu()
     e3.setModel(true)
     obs = []
