@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--strategy", default="auto", choices=("auto", "store", "checkpoint"),
+                    help="forward wavefields for the adjoint: every state kept in HBM, "
+                         "or sqrt(nt) checkpoints with segment re-forward")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every GPU box of this build has one MI355X and the path "
@@ -113,12 +116,13 @@ def main():
         tiles = {k: v["tile"] for k, v in rep.items()}
         print(json.dumps({"autotune": rep}), file=sys.stderr)
 
+    strategy = args.strategy
     for _ in range(args.warmup):
-        prop.gradient(obs, K)
+        prop.gradient(obs, K, strategy)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        J, g, glam, gmu, grho, gm6 = prop.gradient(obs, K)
+        J, g, glam, gmu, grho, gm6 = prop.gradient(obs, K, strategy)
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
     s_per_grad = T / max(args.steps, 1)
@@ -133,7 +137,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": [wl.nx, wl.ny, wl.nz], "nt": nt, "shots": wl.S,
                    "receivers": int(wl.rec.shape[1]), "checkpoint_interval": K,
-                   "gradient": "exact discrete adjoint, sqrt(nt) checkpoints", "tiles": tiles},
+                   "gradient": "exact discrete adjoint", "tiles": tiles,
+                   "forward_wavefields": prop.last_strategy,
+                   "peak_hbm_gb": torch.cuda.max_memory_allocated() / 1e9},
         "s_per_gradient": s_per_grad,
         "forward_grid_pt_updates_per_s": None,
         "misfit": J,

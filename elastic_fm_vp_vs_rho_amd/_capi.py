@@ -52,6 +52,7 @@ _SIGS = {
     "efm_get_tile": ([c_int, P(c_int), P(c_int), P(c_int), P(c_int)], c_int),
     "efm_component_pad": ([], c_int),
     "efm_tile_variants": ([P(c_int), c_int], c_int),
+    "efm_residual": ([c_void_p] * 4 + [c_ll, c_void_p], c_int),
     "efm_model_lame": ([c_void_p] * 5 + [c_ll, c_void_p], c_int),
     "efm_model_lame_adjoint": ([c_void_p] * 9 + [c_ll, c_void_p], c_int),
     "efm_stress_step": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), P(EfmState), c_int,

@@ -496,6 +496,25 @@ __global__ void k_combine(Geo g, const float *mu, const float *mbar, const float
     grho[i] = -0.5f * c * sr;
 }
 
+// r = mod - obs and J += ½Σr² (block partial sums in double, one atomic per block)
+__global__ void k_residual(const float *mod, const float *obs, float *res, double *J, long long n) {
+    __shared__ double part[256];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const float r = mod[i] - obs[i];
+        res[i] = r;
+        v = 0.5 * (double)r * (double)r;
+    }
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(J, part[0]);
+}
+
 __global__ void k_lame(const float *vp, const float *vs, const float *rho, float *lam, float *mu,
                        long long n) {
 #pragma clang fp contract(off)
@@ -614,6 +633,15 @@ extern "C" int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const 
     hipLaunchKernelGGL(k_combine, dim3(blocks_for(g.N, 256)), dim3(256), 0, (hipStream_t)stream, g,
                        m->mu, mbar, bbar, acc, glam, gmu, grho);
     return check_launch("gradient combine");
+}
+
+extern "C" int efm_residual(const float *mod, const float *obs, float *res, double *J, long long n,
+                            void *stream) {
+    if (!mod || !obs || !res || !J || n < 0) return fail(EFM_E_ARG, "efm_residual: bad argument");
+    if (n == 0) return EFM_OK;
+    hipLaunchKernelGGL(k_residual, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, mod,
+                       obs, res, J, n);
+    return check_launch("residual");
 }
 
 extern "C" int efm_model_lame(const float *vp, const float *vs, const float *rho, float *lam,

@@ -2,8 +2,9 @@
 
 ``elastic_modeling(vp, vs, rho, m6, grid, shots)`` returns the gathers
 (S, 3, ntmax, nrec) of a shot batch.  Its backward is the exact discrete
-adjoint of the HIP propagator (adjoint.hip): the forward keeps √nt checkpoints,
-the backward rebuilds each segment and runs the transposed sweeps, which
+adjoint of the HIP propagator (adjoint.hip): the forward keeps every state when
+they fit in HBM (else √nt checkpoints, and the backward rebuilds each
+segment), the backward runs the transposed sweeps, which
 cross-correlate forward and adjoint wavefields into ∂/∂λ, ∂/∂μ, ∂/∂ρ and
 ∂/∂M; efm_model_lame_adjoint maps them to Vp, Vs, ρ.  No torch arithmetic is
 involved on either pass.
@@ -25,17 +26,20 @@ class ElasticModeling(torch.autograd.Function):
         shots.m6.copy_(m6.detach().reshape(shots.S, 6).to(torch.float32))
         lam, mu = lame(vp, vs, rho)
         prop = Propagator(grid, lam, mu, rho, shots)
-        g, snaps, K = prop.forward_checkpointed(K)
-        ctx.prop, ctx.snaps, ctx.K = prop, snaps, K
+        if not any(ctx.needs_input_grad[:4]):
+            return prop.forward()[0]          # no backward will follow: keep nothing
+        tape = prop.record(K, strategy="checkpoint" if K is not None else "auto")
+        ctx.prop, ctx.tape = prop, tape
         ctx.save_for_backward(vp, vs, rho)
         ctx.m6_dtype = m6.dtype
-        return g
+        return tape["gathers"]
 
     @staticmethod
     def backward(ctx, grad_g):
         vp, vs, rho = ctx.saved_tensors
-        glam, gmu, grho, gm6 = ctx.prop.gradient_from_snapshots(ctx.snaps, ctx.K, grad_g)
-        ctx.snaps = None
+        glam, gmu, grho, gm6 = ctx.prop.backprop(ctx.tape, grad_g)
+        ctx.tape = None
+        ctx.prop.release_states()
         gvp, gvs, grho_total = lame_adjoint(vp, vs, rho, glam, gmu, grho)
         return gvp, gvs, grho_total, gm6.to(ctx.m6_dtype), None, None, None
 

@@ -251,23 +251,79 @@ class Propagator:
         return torch.zeros((self.shots.S, self.shots.ntmax, per), dtype=torch.float32,
                            device=self.lam.device)
 
-    # -- exact adjoint with √nt checkpointing ----------------------------------
+    # -- exact adjoint: tape (checkpoints or every state) + reverse sweep ----------
     def checkpoint_interval(self):
         return max(1, int(math.ceil(math.sqrt(self.shots.ntmax))))
 
-    def forward_checkpointed(self, K=None):
-        """Forward run keeping the pre-step state every K steps."""
-        K = self.checkpoint_interval() if K is None else int(K)
+    def state_bytes(self):
+        g, S = self.grid, self.shots.S
+        Lx, Ly, Lz = g.slab_lengths
+        ny, nz, nx = g.shape
+        return 4 * (9 * (S * g.N + 1088) + 6 * S * (ny * nz * Lx + Ly * nz * nx + ny * Lz * nx))
+
+    def choose_strategy(self, budget=None):
+        """'store' keeps every forward state (no re-forward; (nt+1) states of
+        HBM), 'checkpoint' keeps √nt snapshots and rebuilds each segment."""
+        need = (self.shots.ntmax + 1) * self.state_bytes()
+        have = getattr(self, "_pool", [])
+        if len(have) >= self.shots.ntmax + 1:
+            return "store"
+        if budget is None:
+            free, _ = torch.cuda.mem_get_info(self.lam.device)
+            budget = 0.85 * free
+        return "store" if need <= budget else "checkpoint"
+
+    def _states(self, n):
+        """At least n zeroed wavefields, allocated once per propagator and reused
+        (kernels write only interior cells, so the zero borders survive)."""
+        pool = getattr(self, "_pool", None)
+        if pool is None:
+            pool = self._pool = []
+        while len(pool) < n:
+            pool.append(self.new_wavefield())
+        return pool[:n]
+
+    def release_states(self):
+        self._pool = []
+
+    @staticmethod
+    def _array(states):
+        return (ctypes.POINTER(EfmState) * len(states))(*[ctypes.pointer(w.struct()) for w in states])
+
+    def record(self, K=None, strategy="auto"):
+        """Forward run that keeps what the adjoint needs.  Returns a tape dict."""
+        if strategy == "auto":
+            strategy = self.choose_strategy()
         nt = self.shots.ntmax
-        st = self.new_wavefield()
         g = self.new_gathers()
+        if strategy == "store":
+            states = self._states(nt + 1)
+            states[0].zero_()
+            d, m, sh = self.dims(), self.medium(), self.shots.struct()
+            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
+                                              self._array(states), _ptr(g), 0, nt, _stream()),
+                  "efm_forward_states")
+            return dict(strategy="store", gathers=g, states=states)
+        K = self.checkpoint_interval() if K is None else int(K)
+        st = self.new_wavefield()
         snaps = []
         for n0 in range(0, nt, K):
             snaps.append(st.clone())
             self.forward(st, g, it0=n0, nsteps=min(K, nt - n0))
-        return g, snaps, K
+        return dict(strategy="checkpoint", gathers=g, snaps=snaps, K=K)
 
-    def gradient_from_snapshots(self, snaps, K, resid):
+    def misfit(self, gathers, obs):
+        """r = d - obs and J = ½‖r‖² on the device (efm_residual)."""
+        obs = obs.to(device=self.lam.device, dtype=torch.float32).contiguous()
+        if obs.shape != gathers.shape:
+            raise ValueError(f"observed data must be {tuple(gathers.shape)}")
+        r = torch.empty_like(gathers)
+        J = torch.zeros(1, dtype=torch.float64, device=self.lam.device)
+        check(self.lib.efm_residual(_ptr(gathers), _ptr(obs), _ptr(r), _ptr(J), gathers.numel(),
+                                    _stream()), "efm_residual")
+        return r, J
+
+    def backprop(self, tape, resid):
         """∂J/∂λ, ∂J/∂μ, ∂J/∂ρ (volumes) and ∂J/∂M (S, 6) float64 for
         ∂J/∂gathers = resid (S, 3, ntmax, nrec)."""
         s, g = self.shots, self.grid
@@ -281,23 +337,31 @@ class Propagator:
         check(self.lib.efm_derived_medium(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
                                           _stream()), "efm_derived_medium")
         nt = s.ntmax
-        bufs = [self.new_wavefield() for _ in range(min(K, nt) + 1)]
         adj = self.new_wavefield()
         adj2 = Wavefield(g, s.S, dev, psi_only_of=adj)
         acc = padded_components(s.S * 8, g.N, dev)
         gm6 = torch.zeros((s.S, 6), dtype=torch.float64, device=dev)
-        for seg in range(len(snaps) - 1, -1, -1):
-            n0 = seg * K
-            n1 = min(n0 + K, nt)
-            k = n1 - n0
-            bufs[0].copy_(snaps[seg])
-            arr = (ctypes.POINTER(EfmState) * (k + 1))(*[ctypes.pointer(b.struct()) for b in bufs[:k + 1]])
-            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), arr,
-                                              None, n0, k, _stream()), "efm_forward_states")
+
+        def sweep(arr, n0, n1):
             check(self.lib.efm_adjoint(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
                                        ctypes.byref(sh), arr, ctypes.byref(adj.struct()),
                                        ctypes.byref(adj2.struct()), _ptr(resid), _ptr(acc), _ptr(gm6),
                                        n0, n1, int(n0 == 0), _stream()), "efm_adjoint")
+
+        if tape["strategy"] == "store":
+            sweep(self._array(tape["states"][:nt + 1]), 0, nt)
+        else:
+            K, snaps = tape["K"], tape["snaps"]
+            bufs = self._states(min(K, nt) + 1)
+            for seg in range(len(snaps) - 1, -1, -1):
+                n0 = seg * K
+                n1 = min(n0 + K, nt)
+                k = n1 - n0
+                bufs[0].copy_(snaps[seg])
+                arr = self._array(bufs[:k + 1])
+                check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), arr,
+                                                  None, n0, k, _stream()), "efm_forward_states")
+                sweep(arr, n0, n1)
         glam = torch.empty(g.shape, dtype=torch.float32, device=dev)
         gmu = torch.empty_like(glam)
         grho = torch.empty_like(glam)
@@ -306,9 +370,17 @@ class Propagator:
               "efm_gradient_combine")
         return glam, gmu, grho, gm6
 
-    def gradient(self, obs, K=None):
+    def forward_checkpointed(self, K=None):
+        """Forward run keeping the pre-step state every K steps."""
+        t = self.record(K, strategy="checkpoint")
+        return t["gathers"], t["snaps"], t["K"]
+
+    def gradient_from_snapshots(self, snaps, K, resid):
+        return self.backprop(dict(strategy="checkpoint", snaps=snaps, K=K), resid)
+
+    def gradient(self, obs, K=None, strategy="auto"):
         """Misfit J = ½‖d - obs‖² and its exact gradient (the FWI gradient)."""
-        g, snaps, K = self.forward_checkpointed(K)
-        resid = g - obs
-        J = 0.5 * float(torch.sum(resid.double() ** 2))
-        return (J, g) + self.gradient_from_snapshots(snaps, K, resid)
+        tape = self.record(K, strategy)
+        resid, J = self.misfit(tape["gathers"], obs)
+        self.last_strategy = tape["strategy"]
+        return (float(J), tape["gathers"]) + self.backprop(tape, resid)

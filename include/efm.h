@@ -99,6 +99,11 @@ int efm_model_lame_adjoint(const float *vp, const float *vs, const float *rho,
                            const float *glam, const float *gmu, const float *grho_in,
                            float *gvp, float *gvs, float *grho_out, long long n, void *stream);
 
+/* computeResidual / computeCostFunction (M:2128-2142): res = mod - obs
+ * elementwise and J += ½Σres² (J is a device double the caller zeroes). */
+int efm_residual(const float *mod, const float *obs, float *res, double *J, long long n,
+                 void *stream);
+
 /* ---- forward sweeps ---------------------------------------------------- */
 /* kernel_Tau (M:59-128) + addSource (M:1728-1734): τ/ψ_v of `out` from v, τ,
  * ψ_v of `in` (in may alias out); the source term M·w[it] is added to all six
