@@ -126,11 +126,14 @@ def test_exact_adjoint_given_the_oracle_residual(gpu_lib):
     assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < 1e-5
 
 
-def test_checkpoint_interval_does_not_change_the_gradient(gpu_lib):
+def test_forward_wavefield_strategy_does_not_change_the_gradient(gpu_lib):
+    """√nt checkpoints of any interval and keeping every state give identical bits."""
     p = _gradient_problem()
     prop = propagator_of(p)
     obs = torch.zeros((1, 3, p.nt, len(p.rec)), device="cuda")
-    outs = [prop.gradient(obs, K=K) for K in (1, 7, p.nt)]
+    outs = [prop.gradient(obs, K=K, strategy="checkpoint") for K in (1, 7, p.nt)]
+    outs.append(prop.gradient(obs, strategy="store"))
+    assert prop.last_strategy == "store"
     for o in outs[1:]:
         for a, b in zip(outs[0][2:], o[2:]):
             assert torch.equal(a, b)
