@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only run a few dispatches of each sweep kernel and a 256 MiB copy "
+                         "(for hardware-counter runs)")
     ap.add_argument("--strategy", default="auto", choices=("auto", "store", "checkpoint"),
                     help="forward wavefields for the adjoint: every state kept in HBM, "
                          "or sqrt(nt) checkpoints with segment re-forward")
@@ -109,6 +112,17 @@ def main():
     lam, mu = lame(f(wl.vp), f(wl.vs), f(wl.rho))
     prop = Propagator(grid, lam, mu, f(wl.rho), shots)
     K = prop.checkpoint_interval()
+    if args.profile_kernels:
+        from elastic_fm_vp_vs_rho_amd import autotune
+        autotune.tune(prop, use_cache=False, reps=3)
+        kt = time_kernels(prop, reps=4)
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)   # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernels_ms": kt, "copy_bytes_each_way": src.numel() * 4}))
+        return
     tiles = None
     if not args.no_tune:
         from elastic_fm_vp_vs_rho_amd import autotune
