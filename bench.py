@@ -33,6 +33,53 @@ HBM_PEAK_GBS = 8000.0
 #   adj_str    τ̄×6, forward v×3, λ, μ, μ̄×3 read; v̄×3, λμ-sums×5 r+w = 30 words
 ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 120}
 
+# HBM bytes per dispatch on the 128^3 C2 grid, by tile "bx,bz,ny,map": rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors that make the counters read
+# 256 MiB each way for a 256 MiB copy), medians over the dispatches of
+# `bench.py --profile-kernels`.  Only valid for the kernels' current sources and
+# the 128^3 grid; any other size reports traffic null.
+MEASURED_TRAFFIC = {
+    "stress": {
+        "64,4,1,0": 177757695, "64,1,1,0": 208418611, "64,2,1,0": 187284916, "128,1,1,0": 191184953,
+        "128,2,1,0": 171721452, "64,4,2,0": 182782476, "64,1,1,8": 155795268, "64,2,1,8": 155792964,
+        "64,4,1,8": 155797347, "64,8,1,8": 155796419, "128,1,1,8": 155798371, "128,2,1,8": 155796580,
+        "256,1,1,8": 155786949, "64,4,2,8": 155997742,
+    },
+    "velocity": {
+        "64,4,1,0": 145702480, "64,1,1,0": 174881162, "64,2,1,0": 154741211, "128,1,1,0": 159615860,
+        "128,2,1,0": 140186562, "64,4,2,0": 154144990, "64,1,1,8": 124286648, "64,2,1,8": 124293079,
+        "64,4,1,8": 124290871, "64,8,1,8": 124283000, "128,1,1,8": 124293975, "128,2,1,8": 124294263,
+        "256,1,1,8": 124278457, "64,4,2,8": 125408282,
+    },
+    "adj_velocity": {
+        "64,4,1,0": 316624524, "64,1,1,0": 382917928, "64,2,1,0": 337128339, "128,1,1,0": 351860442,
+        "128,2,1,0": 308160329, "64,4,2,0": 380696705, "64,1,1,8": 277730875, "64,2,1,8": 277059078,
+        "64,4,1,8": 276922037, "64,8,1,8": 275311434, "128,1,1,8": 277234002, "128,2,1,8": 277065477,
+        "256,1,1,8": 274539424, "64,4,2,8": 320653905,
+    },
+    "adj_stress": {
+        "64,4,1,0": 329003055, "64,1,1,0": 413224433, "64,2,1,0": 355178170, "128,1,1,0": 376070479,
+        "128,2,1,0": 320266634, "64,4,2,0": 415256462, "64,1,1,8": 283137099, "64,2,1,8": 282731096,
+        "64,4,1,8": 282089440, "64,8,1,8": 282051397, "128,1,1,8": 282760693, "128,2,1,8": 282663584,
+        "256,1,1,8": 279135851, "64,4,2,8": 331864823,
+    },
+}
+
+
+def measured_traffic(kernel, wl):
+    """HBM bytes per launch of ``kernel`` with the installed tiles (C2 grid only)."""
+    from elastic_fm_vp_vs_rho_amd import autotune
+    if (wl.nx, wl.ny, wl.nz) != (128, 128, 128):
+        return None
+    names = ("adj_velocity", "adj_stress") if kernel == "adjoint_step" else (kernel,)
+    total = 0
+    for k in names:
+        key = ",".join(str(v) for v in autotune.get_tile(autotune.KERNELS.index(k)))
+        if key not in MEASURED_TRAFFIC[k]:
+            return None
+        total += MEASURED_TRAFFIC[k][key]
+    return total
+
 
 def _events():
     import torch
@@ -176,7 +223,8 @@ def main():
         else:
             achieved, name, alg = per[dom], dom, ALG_BYTES[dom]
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, wl),
+                            "kernel": name,
                             "alg_bytes_per_cell": alg, "cells_per_launch": interior,
                             "avg_ms": kt[dom], "all_kernels_ms": kt,
                             "all_kernels_gbs": dict(per, adjoint_step=adj_gbs)}
