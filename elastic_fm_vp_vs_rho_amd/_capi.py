@@ -81,6 +81,20 @@ _SIGS = {
                                  c_void_p, c_int, P(EfmState), c_void_p, c_float, c_float, c_float,
                                  c_float, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "efm_border_floats": ([P(EfmDims), c_int], c_ll),
+    # initializer network (include/efm_initnet.h)
+    "efm_net_cast_bf16": ([c_void_p, c_void_p, c_ll, c_void_p], c_int),
+    "efm_net_gemm": ([c_int, c_int, c_int, c_void_p, c_ll, c_ll, c_void_p, c_ll, c_ll, c_void_p, c_ll,
+                      c_void_p, c_void_p, c_int, c_void_p], c_int),
+    "efm_net_im2col": ([c_void_p, c_void_p] + [c_int] * 6 + [c_void_p], c_int),
+    "efm_net_col2im": ([c_void_p, c_void_p] + [c_int] * 6 + [c_void_p], c_int),
+    "efm_net_relu_backward": ([c_void_p] * 4 + [c_ll, c_void_p], c_int),
+    "efm_net_bias_grad": ([c_void_p, c_void_p, c_int, c_ll, c_void_p], c_int),
+    "efm_net_mse": ([c_void_p] * 5 + [c_ll, c_float, c_void_p], c_int),
+    "efm_net_adam": ([c_void_p] * 5 + [c_ll] + [c_float] * 6 + [c_void_p], c_int),
+    "efm_net_flatten": ([c_void_p, c_void_p, c_int, c_int, c_int, c_void_p], c_int),
+    "efm_net_unflatten": ([c_void_p, c_void_p, c_int, c_int, c_int, c_void_p], c_int),
+    "efm_net_pack_gathers": ([c_void_p] + [c_int] * 5 + [c_void_p, c_ll, c_void_p], c_int),
+    "efm_net_gather_batch": ([c_void_p, c_void_p, c_ll, c_void_p] + [c_int] * 4 + [c_void_p] * 3, c_int),
 }
 
 _lock = threading.Lock()

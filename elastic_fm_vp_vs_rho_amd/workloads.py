@@ -167,3 +167,33 @@ def c3(nx=256, ny=256, nz=128, S=32, nt_range=(1500, 3000), nrec=128, seed=4):
     rec = np.broadcast_to(rec1[None], (S,) + rec1.shape).copy()
     return Workload(f"C3 batched {nx}x{ny}x{nz}, {S} shots", nx, ny, nz, dx, dt, W, False, vp, vs,
                     rho, src, m6, f0, nt, rec, vmax)
+
+
+def c5(n=96, S=20000, nt=1024, nrec=64, f0=10.0, seed=5):
+    """C5 initializer data: one 96³ random layered model (seed 5), S sources
+    with uniform random location (x, y, z in [W+4, n-W-4)) and normalised MT,
+    10 Hz Ricker, nt steps, 8×8 receivers one level below the top CPML."""
+    dx, W = 10.0, 10
+    rng = np.random.default_rng(seed)
+    nlay = 5
+    bounds = np.sort(rng.choice(np.arange(12, n - 12), nlay - 1, replace=False))
+    vps = np.sort(rng.uniform(2500.0, 4500.0, nlay))
+    vp_z = np.zeros(n)
+    lo = 0
+    for i, b in enumerate(list(bounds) + [n]):
+        vp_z[lo:b] = vps[i]
+        lo = b
+    vp = np.broadcast_to(vp_z[None, :, None], (n, n, n)).astype(np.float32).copy()
+    vs = (vp / 1.75).astype(np.float32)
+    rho = M.rho_nafe_drake(vp).astype(np.float32)
+    vmax = float(vp.max())
+    dt = courant_dt(dx, vmax)
+    lo_, hi_ = W + 4, n - W - 4
+    src = rng.integers(lo_, hi_, (S, 3))
+    m6 = np.stack([normalized_mt(rng) for _ in range(S)])
+    side = int(np.sqrt(nrec))
+    rec1 = surface_grid(n, n, W, W, nrec // side, side)
+    rec = np.broadcast_to(rec1[None], (S,) + rec1.shape)
+    return Workload(f"C5 initializer data {n}^3, {S} shots x {nt} steps", n, n, n, dx, dt, W, False,
+                    vp, vs, rho, src, m6, np.full(S, f0), np.full(S, nt), rec, vmax,
+                    extra={"src_range": (lo_, hi_)})

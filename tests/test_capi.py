@@ -1,5 +1,5 @@
-"""The C ABI library loads, exports every symbol include/efm.h declares, and
-rejects bad arguments before touching a GPU (CPU-only test)."""
+"""The C ABI library loads, exports every symbol the headers under include/
+declare, and rejects bad arguments before touching a GPU (CPU-only test)."""
 import ctypes
 import os
 import re
@@ -10,10 +10,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    with open(os.path.join(ROOT, "include", "efm.h")) as f:
-        src = f.read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(efm_[a-z_0-9]+)\s*\(", src)))
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for h in sorted(os.listdir(inc)):
+        if h.endswith(".h"):
+            with open(os.path.join(inc, h)) as f:
+                src = re.sub(r"/\*.*?\*/", "", f.read(), flags=re.S)
+            names |= set(re.findall(r"\b(efm_[a-z_0-9]+)\s*\(", src))
+    return sorted(names)
 
 
 @pytest.fixture(scope="module")
@@ -51,6 +55,10 @@ def test_version_and_errors(L):
     assert all(buf[4 * i] % 64 == 0 for i in range(n))   # whole 64-lane waves along x
     assert {buf[4 * i + 3] for i in range(n)} == {0, 8}   # linear and XCD-aware mappings
     assert L.efm_component_pad() == 1088
+    # initializer kernels validate before launching
+    assert L.efm_net_gemm(0, 1, 1, None, 0, 0, None, 0, 0, None, 1, None, None, 0, None) == -1
+    assert "efm_net_gemm" in _capi.last_error()
+    assert L.efm_net_im2col(None, None, 1, 1, 3, 5, 1, 0, None) == -1
 
 
 def test_argument_validation_without_gpu(L):

@@ -1,0 +1,164 @@
+"""The initializer network's HIP kernels against the FP64 restatement
+(oracle/initnet_fp64.py, BF16 rounding emulated where the device rounds).
+
+Tolerances: a single GEMM 1e-5 (FP32 against FP64 accumulation of the same
+BF16 operands); whole forward/backward passes 2e-3 relative L2 per tensor (FP32
+accumulation plus BF16 rounding flips of activations that sit on a rounding
+boundary); one Adam step 1e-6."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import initnet_fp64 as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(a, dev="cuda"):
+    return torch.as_tensor(np.asarray(a, np.float32), device=dev).to(torch.bfloat16)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 32, False, False), (70, 130, 45, False, False),
+                                         (9, 5, 300, True, False), (33, 257, 16, False, True),
+                                         (128, 17, 1000, True, True), (1, 1, 1, False, False)])
+def test_gemm_against_fp64(gpu_lib, M, N, K, ta, tb):
+    """C = A·B for row-major and transposed operand views, ragged edges, and the
+    bias/ReLU epilogues (fixes the MFMA lane layout: a transposed or permuted
+    tile fails by O(1))."""
+    from elastic_fm_vp_vs_rho_amd.propagator import _ptr, _stream
+    from elastic_fm_vp_vs_rho_amd import _capi
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = R.bf16(rng.standard_normal((M, K)))
+    B = R.bf16(rng.standard_normal((K, N)))
+    bias = rng.standard_normal(M).astype(np.float32)
+    Ad = _bf(A.T.copy() if ta else A)
+    Bd = _bf(B.T.copy() if tb else B)
+    sam, sak = (1, M) if ta else (K, 1)
+    sbk, sbn = (1, K) if tb else (N, 1)
+    want = A @ B
+    for epi in (0, 1, 2):
+        C = torch.full((M, N + 3), 7.0, device="cuda")
+        Cb = torch.zeros((M, N + 3), dtype=torch.bfloat16, device="cuda")
+        bd = torch.as_tensor(bias, device="cuda")
+        _capi.check(gpu_lib.efm_net_gemm(M, N, K, _ptr(Ad), sam, sak, _ptr(Bd), sbk, sbn, _ptr(C), N + 3,
+                                         _ptr(bd), _ptr(Cb), epi, _stream()), "gemm")
+        torch.cuda.synchronize()
+        got = C.cpu().numpy()
+        w = want + (bias[:, None] if epi else 0.0)
+        if epi == 1:
+            w = np.maximum(w, 0.0)
+        assert _rel(got[:, :N], w) < 1e-5, epi
+        assert np.all(got[:, N:] == 7.0)                      # nothing written past N
+        np.testing.assert_array_equal(Cb[:, :N].float().cpu().numpy(), R.bf16(got[:, :N]))
+
+
+def test_im2col_col2im_flatten_kernels(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd.propagator import _ptr, _stream
+    from elastic_fm_vp_vs_rho_amd import _capi
+    rng = np.random.default_rng(3)
+    C, B, L, ks, s, p = 5, 3, 37, 9, 4, 4
+    lo = (L + 2 * p - ks) // s + 1
+    x = R.bf16(rng.standard_normal((C, B, L)))
+    xd = _bf(x)
+    col = torch.zeros((C * ks, B * lo), dtype=torch.bfloat16, device="cuda")
+    _capi.check(gpu_lib.efm_net_im2col(_ptr(xd), _ptr(col), C, B, L, ks, s, p, _stream()), "im2col")
+    d = rng.standard_normal((C * ks, B * lo)).astype(np.float32)
+    dx = torch.zeros((C, B, L), device="cuda")
+    _capi.check(gpu_lib.efm_net_col2im(_ptr(torch.as_tensor(d, device="cuda")), _ptr(dx), C, B, L, ks, s, p,
+                                       _stream()), "col2im")
+    X = torch.zeros((C * L, B), dtype=torch.bfloat16, device="cuda")
+    _capi.check(gpu_lib.efm_net_flatten(_ptr(xd), _ptr(X), C, B, L, _stream()), "flatten")
+    dX = rng.standard_normal((C * L, B)).astype(np.float32)
+    da = torch.zeros((C, B, L), device="cuda")
+    _capi.check(gpu_lib.efm_net_unflatten(_ptr(torch.as_tensor(dX, device="cuda")), _ptr(da), C, B, L,
+                                          _stream()), "unflatten")
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(col.float().cpu().numpy(), R.im2col(x, ks, s, p))
+    assert _rel(dx.cpu().numpy(), R.col2im(d.astype(np.float64), C, B, L, ks, s, p)) < 1e-6
+    np.testing.assert_array_equal(X.float().cpu().numpy(), x.transpose(0, 2, 1).reshape(C * L, B))
+    np.testing.assert_array_equal(da.cpu().numpy(), dX.reshape(C, L, B).transpose(0, 2, 1))
+
+
+SMALL = dict(cin=12, length=200, channels=16, ks=5, stride=2, pad=2, nblocks=3, hidden=32, nout=9)
+
+
+def test_forward_backward_against_oracle(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd.initnet import InitNet, NetSpec
+    spec = NetSpec(**SMALL)
+    ospec = R.Spec(**SMALL)
+    rng = np.random.default_rng(11)
+    B = 6
+    p = R.init_params(ospec, seed=2)
+    for k in p:
+        if k[0] in "bc":
+            p[k] = (0.05 * rng.standard_normal(p[k].shape)).astype(np.float32)
+    net = InitNet(spec, params=p, device="cuda")
+    x = R.bf16(rng.standard_normal((spec.cin, B, spec.length)))
+    t = rng.standard_normal((spec.nout, B)).astype(np.float32)
+    net.set_input(_bf(x))
+    net._buffers(B)["t"].copy_(torch.as_tensor(t, device="cuda"))
+    out = net.forward_buffers(B)
+    loss = net.backward_buffers(B)
+    torch.cuda.synchronize()
+    ro, cache = R.forward(ospec, p, x, emulate=True)
+    rl, rg = R.backward(ospec, p, cache, t.astype(np.float64), emulate=True)
+    assert _rel(out.cpu().numpy(), ro) < 2e-3
+    assert abs(float(loss) - rl) <= 2e-3 * rl
+    for k, g in rg.items():
+        e = _rel(net.g[k].cpu().numpy(), g)
+        assert e < 2e-3, (k, e)
+    # one Adam step against the restatement, from the device gradients
+    w0 = {k: v.cpu().numpy().astype(np.float64) for k, v in net.w.items()}
+    gd = {k: v.cpu().numpy().astype(np.float64) for k, v in net.g.items()}
+    net.adam(lr=1e-3)
+    torch.cuda.synchronize()
+    for k in w0:
+        w1, _, _ = R.adam(w0[k], gd[k], np.zeros_like(w0[k]), np.zeros_like(w0[k]), 1, lr=1e-3)
+        np.testing.assert_allclose(net.w[k].cpu().numpy(), w1, rtol=0, atol=1e-6)
+        np.testing.assert_array_equal(net.wb[k].float().cpu().numpy(), R.bf16(net.w[k].cpu().numpy()))
+
+
+def test_pack_and_gather_batch(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd.initnet import InitNet, NetSpec, pack_gathers
+    rng = np.random.default_rng(5)
+    S, nrec, nt, L = 5, 4, 40, 32
+    g = rng.standard_normal((S, 3, nt, nrec)).astype(np.float32)
+    g[2] *= 1e-5
+    data = torch.zeros((S, 3 * nrec, L), dtype=torch.bfloat16, device="cuda")
+    pack_gathers(torch.as_tensor(g, device="cuda"), L, data, 0)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(data.float().cpu().numpy(), R.pack_gathers(g, L))
+    spec = NetSpec(cin=3 * nrec, length=L, channels=8, ks=3, stride=2, pad=1, nblocks=2, hidden=8)
+    net = InitNet(spec, device="cuda")
+    tg = torch.as_tensor(rng.standard_normal((S, 9)).astype(np.float32), device="cuda")
+    idx = torch.tensor([3, 0, 4], dtype=torch.int32, device="cuda")
+    B = net.load_batch(data, tg, idx)
+    b = net._buffers(B)
+    torch.cuda.synchronize()
+    assert torch.equal(b["x"], data[idx.long()].permute(1, 0, 2))
+    assert torch.equal(b["t"], tg[idx.long()].t())
+
+
+def test_training_on_propagator_data_learns(gpu_lib):
+    """Small C5-like problem (40³, 384 shots, 256 steps, 16 receivers): the
+    network trained by the HIP kernels predicts held-out source parameters
+    better than the training mean."""
+    from elastic_fm_vp_vs_rho_amd import initnet_data as D
+    from elastic_fm_vp_vs_rho_amd import workloads
+    from elastic_fm_vp_vs_rho_amd.initnet import InitNet, NetSpec
+    wl = workloads.c5(n=40, S=384, nt=256, nrec=16)
+    c5 = D.C5(wl, "cuda")
+    data, tg, _ = c5.generate(np.arange(384), 256, batch=128)
+    spec = NetSpec(cin=48, length=256, channels=32, ks=9, stride=4, pad=4, nblocks=3, hidden=64)
+    net = InitNet(spec, seed=1, device="cuda")
+    hist = D.train(net, data, tg, np.arange(64, 384), np.arange(64), epochs=30, batch=32, lr=2e-3)
+    va = torch.arange(64, device="cuda")
+    base = float(((tg[:64] - tg[64:].mean(0)) ** 2).mean())
+    assert all(np.isfinite(h).all() for h in hist)
+    assert hist[-1][0] < 0.5 * hist[0][0]
+    assert hist[-1][1] < 0.8 * base, (hist[-1], base)
