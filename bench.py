@@ -40,28 +40,28 @@ ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 12
 # the 128^3 grid; any other size reports traffic null.
 MEASURED_TRAFFIC = {
     "stress": {
-        "64,4,1,0": 177757695, "64,1,1,0": 208418611, "64,2,1,0": 187284916, "128,1,1,0": 191184953,
-        "128,2,1,0": 171721452, "64,4,2,0": 182782476, "64,1,1,8": 155795268, "64,2,1,8": 155792964,
-        "64,4,1,8": 155797347, "64,8,1,8": 155796419, "128,1,1,8": 155798371, "128,2,1,8": 155796580,
-        "256,1,1,8": 155786949, "64,4,2,8": 155997742,
+        "64,4,1,0": 177758600, "64,1,1,0": 208423755, "64,2,1,0": 187290808, "128,1,1,0": 191204195,
+        "128,2,1,0": 171726402, "64,4,2,0": 183123444, "64,1,1,8": 155798283, "64,2,1,8": 155796939,
+        "64,4,1,8": 155802026, "64,8,1,8": 155798123, "128,1,1,8": 155804650, "128,2,1,8": 155799915,
+        "256,1,1,8": 155788332, "64,4,2,8": 156076752,
     },
     "velocity": {
-        "64,4,1,0": 145702480, "64,1,1,0": 174881162, "64,2,1,0": 154741211, "128,1,1,0": 159615860,
-        "128,2,1,0": 140186562, "64,4,2,0": 154144990, "64,1,1,8": 124286648, "64,2,1,8": 124293079,
-        "64,4,1,8": 124290871, "64,8,1,8": 124283000, "128,1,1,8": 124293975, "128,2,1,8": 124294263,
-        "256,1,1,8": 124278457, "64,4,2,8": 125408282,
+        "64,4,1,0": 145703531, "64,1,1,0": 174883745, "64,2,1,0": 154744812, "128,1,1,0": 159619191,
+        "128,2,1,0": 140187058, "64,4,2,0": 154390543, "64,1,1,8": 124288216, "64,2,1,8": 124294520,
+        "64,4,1,8": 124292600, "64,8,1,8": 124284921, "128,1,1,8": 124295160, "128,2,1,8": 124298039,
+        "256,1,1,8": 124279513, "64,4,2,8": 125238427,
     },
     "adj_velocity": {
-        "64,4,1,0": 316624524, "64,1,1,0": 382917928, "64,2,1,0": 337128339, "128,1,1,0": 351860442,
-        "128,2,1,0": 308160329, "64,4,2,0": 380696705, "64,1,1,8": 277730875, "64,2,1,8": 277059078,
-        "64,4,1,8": 276922037, "64,8,1,8": 275311434, "128,1,1,8": 277234002, "128,2,1,8": 277065477,
-        "256,1,1,8": 274539424, "64,4,2,8": 320653905,
+        "64,4,1,0": 327312657, "64,1,1,0": 391132481, "64,2,1,0": 348036013, "128,1,1,0": 362313621,
+        "128,2,1,0": 318771747, "64,4,2,0": 374315746, "64,1,1,8": 298119073, "64,2,1,8": 297790657,
+        "64,4,1,8": 297088519, "64,8,1,8": 296897178, "128,1,1,8": 298098627, "128,2,1,8": 296407978,
+        "256,1,1,8": 298547286, "64,4,2,8": 316725903,
     },
     "adj_stress": {
-        "64,4,1,0": 329003055, "64,1,1,0": 413224433, "64,2,1,0": 355178170, "128,1,1,0": 376070479,
-        "128,2,1,0": 320266634, "64,4,2,0": 415256462, "64,1,1,8": 283137099, "64,2,1,8": 282731096,
-        "64,4,1,8": 282089440, "64,8,1,8": 282051397, "128,1,1,8": 282760693, "128,2,1,8": 282663584,
-        "256,1,1,8": 279135851, "64,4,2,8": 331864823,
+        "64,4,1,0": 338542509, "64,1,1,0": 421641960, "64,2,1,0": 365565849, "128,1,1,0": 386331231,
+        "128,2,1,0": 329850287, "64,4,2,0": 407310196, "64,1,1,8": 300014650, "64,2,1,8": 299700921,
+        "64,4,1,8": 298999615, "64,8,1,8": 299529355, "128,1,1,8": 300260642, "128,2,1,8": 298463155,
+        "256,1,1,8": 299857129, "64,4,2,8": 328074528,
     },
 }
 
@@ -94,6 +94,52 @@ def time_kernels(prop, reps=30):
     from elastic_fm_vp_vs_rho_amd.autotune import _Bench
     B = _Bench(prop, warm_steps=100)
     return {"stress": B.time(0, reps), "velocity": B.time(1, reps), "adjoint_step": B.time(2, reps)}
+
+
+def batched_vs_plain(wl, S, steps, chunk=50):
+    """Forward grid-point updates/s on a bounded sample of a multi-shot
+    workload: its first S shots (record lengths scaled to at most ``steps``,
+    staying uneven), advanced together in one batch (hipGraph time loop) and
+    one shot at a time (eager launches)."""
+    import numpy as np
+    import torch
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch, lame
+    dev = torch.device("cuda:0")
+    idx = np.arange(S)
+    nt = np.maximum(1, wl.nt[idx] * steps // wl.nt[idx].max())
+    grid = Grid(wl.nx, wl.ny, wl.nz, wl.dx, wl.dt, (wl.W,) * 6)
+    cp = {k: v[idx] for k, v in wl.cpml().items()}
+    wav = wl.wavelets()[idx][:, :int(nt.max())]
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    rho = f(wl.rho)
+    lam, mu = lame(f(wl.vp), f(wl.vs), rho)
+
+    def prop(sel):
+        sh = ShotBatch(grid, wl.src[idx][sel], wl.m6[idx][sel], wav[sel], nt[sel], wl.rec[idx][sel],
+                       {k: v[sel] for k, v in cp.items()}, device=dev)
+        return Propagator(grid, lam, mu, rho, sh)
+
+    batch = prop(np.arange(S))
+    batch.forward(nsteps=chunk, graph_chunk=chunk)           # warm-up (graph capture path)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.forward(graph_chunk=chunk)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    del batch
+    singles = [prop(np.array([s])) for s in range(S)]
+    singles[0].forward(nsteps=chunk)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in singles:
+        p.forward()
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    work = float(grid.N) * float(nt.sum())
+    return {"sample": f"{wl.nx}x{wl.ny}x{wl.nz}, {S} shots, uneven record lengths <= {steps} steps "
+                      f"({int(nt.sum())} shot-steps)",
+            "batched_updates_per_s": work / tb, "plain_updates_per_s": work / tp,
+            "batched_over_plain": tp / tb}
 
 
 def cpu_baseline(wl, steps):
@@ -129,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-vs-plain forward comparison on a C3 sample")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only run a few dispatches of each sweep kernel and a 256 MiB copy "
                          "(for hardware-counter runs)")
@@ -219,7 +267,8 @@ def main():
         adj_gbs = (ALG_BYTES["adj_velocity"] + ALG_BYTES["adj_stress"]) * interior / (kt["adjoint_step"] * 1e-3) / 1e9
         dom = max(kt, key=lambda k: kt[k])
         if dom == "adjoint_step":
-            achieved, name, alg = adj_gbs, "adjoint step (adj_velocity + adj_stress + points)", 240
+            achieved, name = adj_gbs, "adjoint step (adj_velocity + adj_stress + points)"
+            alg = ALG_BYTES["adj_velocity"] + ALG_BYTES["adj_stress"]
         else:
             achieved, name, alg = per[dom], dom, ALG_BYTES[dom]
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -228,6 +277,12 @@ def main():
                             "alg_bytes_per_cell": alg, "cells_per_launch": interior,
                             "avg_ms": kt[dom], "all_kernels_ms": kt,
                             "all_kernels_gbs": dict(per, adjoint_step=adj_gbs)}
+    prop.release_states()
+    if not args.no_batched:
+        c5 = workloads.c5(S=64, nt=256)
+        c5.nt = np.array([128 + (37 * s) % 129 for s in range(c5.S)])      # uneven, 128..256
+        line["batched_vs_plain"] = {"C3": batched_vs_plain(workloads.c3(), 8, 300),
+                                    "C5": batched_vs_plain(c5, 64, int(c5.nt.max()))}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
     print(json.dumps(line))

@@ -25,6 +25,11 @@
 
 namespace efm {
 
+// Loads of the stored forward state carry the streaming (nontemporal) hint:
+// each value is read by one step of the reverse sweep only, and the hint keeps
+// the reused adjoint state, medium and accumulators cached.
+#define LDF(p) __builtin_nontemporal_load(p)
+
 struct AdjArgs {
     const float *lam, *mu;
     const float *mbar, *bbar;      // (3, N) each
@@ -178,32 +183,32 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
         tb5[t] += (hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p);
 
         // ρ accumulators: forward F at p from τ^{n+1} (= τ'' of step n) and ψτ^n
-        const float h_xx = T0[t + 1] - T0[t];
-        const float h_yy = T1[t + sxz] - T1[t];
-        const float h_zz = T2[t + nx] - T2[t];
-        const float t3 = T3[t], t4 = T4[t], t5 = T5[t];
-        const float h_xyy = t3 - T3[t - sxz], h_xyx = t3 - T3[t - 1];
-        const float h_xzz = t4 - T4[t - nx], h_xzx = t4 - T4[t - 1];
-        const float h_yzz = t5 - T5[t - nx], h_yzy = t5 - T5[t - sxz];
+        const float h_xx = LDF(&T0[t + 1]) - LDF(&T0[t]);
+        const float h_yy = LDF(&T1[t + sxz]) - LDF(&T1[t]);
+        const float h_zz = LDF(&T2[t + nx]) - LDF(&T2[t]);
+        const float t3 = LDF(&T3[t]), t4 = LDF(&T4[t]), t5 = LDF(&T5[t]);
+        const float h_xyy = t3 - LDF(&T3[t - sxz]), h_xyx = t3 - LDF(&T3[t - 1]);
+        const float h_xzz = t4 - LDF(&T4[t - nx]), h_xzx = t4 - LDF(&T4[t - 1]);
+        const float h_yzz = t5 - LDF(&T5[t - nx]), h_yzy = t5 - LDF(&T5[t - sxz]);
         float f_xx = h_xx, f_yy = h_yy, f_zz = h_zz, f_xyy = h_xyy, f_xyx = h_xyx;
         float f_xzz = h_xzz, f_xzx = h_xzx, f_yzz = h_yzz, f_yzy = h_yzy;
         if (sx0 >= 0) {
             const int pi = px_off(g, y, z, sx0) + 3 * csx;
-            f_xx += bx0 * Fx[pi] + ax0 * h_xx;
-            f_xyx += bx0 * Fx[pi + csx] + ax0 * h_xyx;
-            f_xzx += bx0 * Fx[pi + 2 * csx] + ax0 * h_xzx;
+            f_xx += bx0 * LDF(&Fx[pi]) + ax0 * h_xx;
+            f_xyx += bx0 * LDF(&Fx[pi + csx]) + ax0 * h_xyx;
+            f_xzx += bx0 * LDF(&Fx[pi + 2 * csx]) + ax0 * h_xzx;
         }
         if (sy0 >= 0) {
             const int pi = py_off(g, sy0, z, x) + 3 * csy;
-            f_xyy += by0 * Fy[pi] + ay0 * h_xyy;
-            f_yy += by0 * Fy[pi + csy] + ay0 * h_yy;
-            f_yzy += by0 * Fy[pi + 2 * csy] + ay0 * h_yzy;
+            f_xyy += by0 * LDF(&Fy[pi]) + ay0 * h_xyy;
+            f_yy += by0 * LDF(&Fy[pi + csy]) + ay0 * h_yy;
+            f_yzy += by0 * LDF(&Fy[pi + 2 * csy]) + ay0 * h_yzy;
         }
         if (sz0 >= 0) {
             const int pi = pz_off(g, y, sz0, x) + 3 * csz;
-            f_xzz += bz0 * Fz[pi] + az0 * h_xzz;
-            f_yzz += bz0 * Fz[pi + csz] + az0 * h_yzz;
-            f_zz += bz0 * Fz[pi + 2 * csz] + az0 * h_zz;
+            f_xzz += bz0 * LDF(&Fz[pi]) + az0 * h_xzz;
+            f_yzz += bz0 * LDF(&Fz[pi + csz]) + az0 * h_yzz;
+            f_zz += bz0 * LDF(&Fz[pi + 2 * csz]) + az0 * h_zz;
         }
         acc5[t] += vx0 * (f_xx + f_xyy + f_xzz);
         acc6[t] += vy0 * (f_xyx + f_yy + f_yzz);
@@ -371,30 +376,30 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_stress(Geo g, AdjArgs A, int
         vb2[t] += (g_vzz_0 - g_vzz_p) + (g_vzx_m - g_vzx_0) + (g_vzy_m - g_vzy_0);
 
         // λ, μ accumulators: forward e at p from v^n and ψv^n
-        const float v0 = V0[t], v1 = V1[t], v2 = V2[t];
-        const float d_xx = v0 - V0[t - 1], d_yy = v1 - V1[t - sxz], d_zz = v2 - V2[t - nx];
-        const float d_xz = V0[t + nx] - v0, d_xy = V0[t + sxz] - v0;
-        const float d_yx = V1[t + 1] - v1, d_yz = V1[t + nx] - v1;
-        const float d_zx = V2[t + 1] - v2, d_zy = V2[t + sxz] - v2;
+        const float v0 = LDF(&V0[t]), v1 = LDF(&V1[t]), v2 = LDF(&V2[t]);
+        const float d_xx = v0 - LDF(&V0[t - 1]), d_yy = v1 - LDF(&V1[t - sxz]), d_zz = v2 - LDF(&V2[t - nx]);
+        const float d_xz = LDF(&V0[t + nx]) - v0, d_xy = LDF(&V0[t + sxz]) - v0;
+        const float d_yx = LDF(&V1[t + 1]) - v1, d_yz = LDF(&V1[t + nx]) - v1;
+        const float d_zx = LDF(&V2[t + 1]) - v2, d_zy = LDF(&V2[t + sxz]) - v2;
         float fxx = d_xx, fyy = d_yy, fzz = d_zz, fxz = d_xz, fxy = d_xy;
         float fyx = d_yx, fyz = d_yz, fzx = d_zx, fzy = d_zy;
         if (sx0 >= 0) {
             const int pi = px_off(g, y, z, sx0);
-            fxx += bx0 * Fx[pi] + ax0 * d_xx;
-            fyx += bx0 * Fx[pi + csx] + ax0 * d_yx;
-            fzx += bx0 * Fx[pi + 2 * csx] + ax0 * d_zx;
+            fxx += bx0 * LDF(&Fx[pi]) + ax0 * d_xx;
+            fyx += bx0 * LDF(&Fx[pi + csx]) + ax0 * d_yx;
+            fzx += bx0 * LDF(&Fx[pi + 2 * csx]) + ax0 * d_zx;
         }
         if (sy0 >= 0) {
             const int pi = py_off(g, sy0, z, x);
-            fxy += by0 * Fy[pi] + ay0 * d_xy;
-            fyy += by0 * Fy[pi + csy] + ay0 * d_yy;
-            fzy += by0 * Fy[pi + 2 * csy] + ay0 * d_zy;
+            fxy += by0 * LDF(&Fy[pi]) + ay0 * d_xy;
+            fyy += by0 * LDF(&Fy[pi + csy]) + ay0 * d_yy;
+            fzy += by0 * LDF(&Fy[pi + 2 * csy]) + ay0 * d_zy;
         }
         if (sz0 >= 0) {
             const int pi = pz_off(g, y, sz0, x);
-            fxz += bz0 * Fz[pi] + az0 * d_xz;
-            fyz += bz0 * Fz[pi + csz] + az0 * d_yz;
-            fzz += bz0 * Fz[pi + 2 * csz] + az0 * d_zz;
+            fxz += bz0 * LDF(&Fz[pi]) + az0 * d_xz;
+            fyz += bz0 * LDF(&Fz[pi + csz]) + az0 * d_yz;
+            fzz += bz0 * LDF(&Fz[pi + 2 * csz]) + az0 * d_zz;
         }
         acc0[t] += (txx + tyy + tzz) * (fxx + fyy + fzz);
         acc1[t] += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
@@ -442,16 +447,18 @@ __global__ void k_derived(Geo g, const float *mu, const float *rho, float *mbar,
     float mxy = 0.f, mxz = 0.f, myz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     if (interior(x, y, z, g)) {
         const long long nx = g.nx, sxz = g.sxz;
-        const float q0 = 0.25f / mu[i], qi = 0.25f / mu[i + 1], qk = 0.25f / mu[i + nx];
-        const float qj = 0.25f / mu[i + sxz], qik = 0.25f / mu[i + 1 + nx];
-        const float qjk = 0.25f / mu[i + nx + sxz], qij = 0.25f / mu[i + 1 + sxz];
-        mxz = 1.f / (q0 + qi + qk + qik);
-        mxy = 1.f / (q0 + qi + qj + qij);
-        myz = 1.f / (q0 + qj + qk + qjk);
+        // the forward sweeps' own operations (forward.hip), so that the adjoint
+        // is the transpose of the forward operator as implemented
+        const float q0 = frcp(mu[i]), qi = frcp(mu[i + 1]), qk = frcp(mu[i + nx]);
+        const float qj = frcp(mu[i + sxz]), qik = frcp(mu[i + 1 + nx]);
+        const float qjk = frcp(mu[i + nx + sxz]), qij = frcp(mu[i + 1 + sxz]);
+        mxz = 4.f * frcp(q0 + qi + qk + qik);
+        mxy = 4.f * frcp(q0 + qi + qj + qij);
+        myz = 4.f * frcp(q0 + qj + qk + qjk);
         const float r = rho[i];
-        bx = 2.f / (r + rho[i + 1]);
-        by = 2.f / (r + rho[i + sxz]);
-        bz = 2.f / (r + rho[i + nx]);
+        bx = 2.f * frcp(r + rho[i + 1]);
+        by = 2.f * frcp(r + rho[i + sxz]);
+        bz = 2.f * frcp(r + rho[i + nx]);
     }
     mbar[i] = mxy; mbar[i + g.ps] = mxz; mbar[i + 2 * g.ps] = myz;
     bbar[i] = bx; bbar[i + g.ps] = by; bbar[i + 2 * g.ps] = bz;

@@ -285,10 +285,21 @@ class Propagator:
 
     def release_states(self):
         self._pool = []
+        self._arrays = {}
 
-    @staticmethod
-    def _array(states):
-        return (ctypes.POINTER(EfmState) * len(states))(*[ctypes.pointer(w.struct()) for w in states])
+    def _array(self, states):
+        """ctypes array of EfmState pointers for `states` (cached per list of
+        wavefields: building it for thousands of states costs milliseconds)."""
+        key = (len(states),) + tuple(id(w) for w in states[:1] + states[-1:])
+        cache = getattr(self, "_arrays", None)
+        if cache is None:
+            cache = self._arrays = {}
+        hit = cache.get(key)
+        if hit is not None and hit[0] == [id(w) for w in states]:
+            return hit[1]
+        arr = (ctypes.POINTER(EfmState) * len(states))(*[ctypes.pointer(w.struct()) for w in states])
+        cache[key] = ([id(w) for w in states], arr, states)
+        return arr
 
     def record(self, K=None, strategy="auto"):
         """Forward run that keeps what the adjoint needs.  Returns a tape dict."""
@@ -383,4 +394,5 @@ class Propagator:
         tape = self.record(K, strategy)
         resid, J = self.misfit(tape["gathers"], obs)
         self.last_strategy = tape["strategy"]
-        return (float(J), tape["gathers"]) + self.backprop(tape, resid)
+        grads = self.backprop(tape, resid)       # queued before J is read back (no host sync)
+        return (float(J), tape["gathers"]) + grads
