@@ -197,3 +197,35 @@ def c5(n=96, S=20000, nt=1024, nrec=64, f0=10.0, seed=5):
     return Workload(f"C5 initializer data {n}^3, {S} shots x {nt} steps", n, n, n, dx, dt, W, False,
                     vp, vs, rho, src, m6, np.full(S, f0), np.full(S, nt), rec, vmax,
                     extra={"src_range": (lo_, hi_)})
+
+
+def c4(nx=200, ny=200, nz=100, S=16, nt=2500, nrec=64, f0=10.0, anomalies=True, seed=6):
+    """C4 FWI + MT: smoothly layered model (Vp 2600 -> 4200 m/s with depth,
+    Vs = Vp/1.73, Gardner ρ) with two Gaussian anomalies (+8 % and -8 % on Vp
+    and Vs; σ = 8 cells) in the true model; S sources at 0.3-0.7 of the depth
+    with normalised MT (seed 6), 10 Hz Ricker, nt steps, 8×8 receivers one
+    level below the top CPML; 10-cell CPML on all faces."""
+    dx, W = 10.0, 10
+    rng = np.random.default_rng(seed)
+    vp_z = np.interp(np.arange(nz), [0, nz // 4, nz // 2, 3 * nz // 4, nz - 1],
+                     [2600.0, 2900.0, 3300.0, 3800.0, 4200.0])
+    vp = np.broadcast_to(vp_z[None, :, None], (ny, nz, nx)).astype(np.float64).copy()
+    if anomalies:
+        yy, zz, xx = np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij")
+        for (cx, cy, cz, amp) in ((0.35 * nx, 0.4 * ny, 0.55 * nz, 0.08), (0.65 * nx, 0.6 * ny, 0.45 * nz, -0.08)):
+            vp *= 1 + amp * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / (2 * 8.0 ** 2))
+    vs = vp / 1.73
+    rho = 310.0 * np.broadcast_to(vp_z[None, :, None], (ny, nz, nx)) ** 0.25
+    vmax = 4200.0 * 1.1
+    dt = courant_dt(dx, vmax)
+    mx, my = max(2, min(10, (nx - 2 * W) // 4)), max(2, min(10, (ny - 2 * W) // 4))
+    zlo, zhi = max(W + 2, int(0.3 * nz)), min(nz - W - 2, int(0.7 * nz) + 1)
+    src = np.stack([rng.integers(W + mx, nx - W - mx, S), rng.integers(W + my, ny - W - my, S),
+                    rng.integers(zlo, zhi, S)], 1)
+    m6 = np.stack([normalized_mt(rng) for _ in range(S)])
+    side = int(np.sqrt(nrec))
+    rec1 = surface_grid(nx, ny, W, W, nrec // side, side)
+    rec = np.broadcast_to(rec1[None], (S,) + rec1.shape).copy()
+    name = f"C4 FWI+MT {nx}x{ny}x{nz}, {S} shots x {nt} steps" + (" (true model)" if anomalies else "")
+    return Workload(name, nx, ny, nz, dx, dt, W, False, vp.astype(np.float32), vs.astype(np.float32),
+                    rho.astype(np.float32), src, m6, np.full(S, f0), np.full(S, nt), rec, vmax)
