@@ -90,8 +90,8 @@ class ShotBatch:
         lin = np.where(used, rx + grid.nx * rz + grid.nx * grid.nz * ry, -1)
         self.S, self.ntmax, self.nrec = S, wavelet.shape[1], rec.shape[1]
         dev = torch.device(device)
-        f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
-        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.int32), device=dev)
+        f32 = lambda a: torch.as_tensor(np.array(a, np.float32), device=dev)
+        i32 = lambda a: torch.as_tensor(np.array(a, np.int32), device=dev)
         self.src_host, self.nt_host, self.rec_host = src, nt, rec
         self.src, self.m6, self.wavelet = i32(src), f32(np.asarray(m6).reshape(S, 6)), f32(wavelet)
         self.nt, self.rec = i32(nt), i32(lin)
