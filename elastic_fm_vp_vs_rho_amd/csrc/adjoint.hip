@@ -585,7 +585,8 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
                            const float *bbar, const EfmShots *sh, const EfmState *const *fwd,
                            const EfmState *adj, const EfmState *adj2, const float *resid,
                            float *acc, double *gm6, int n0, int n1, int last, void *stream) {
-    if (!d || !m || !mbar || !bbar || !sh || !fwd || !adj || !adj2 || !resid || !acc || !gm6)
+    if (!d || !m || !mbar || !bbar || !sh || !fwd || !adj || !adj2 || (!resid && d->nrec > 0) ||
+        !acc || !gm6)
         return fail(EFM_E_ARG, "efm_adjoint: null argument");
     if (n0 < 0 || n1 <= n0 || n1 > d->ntmax) return fail(EFM_E_ARG, "efm_adjoint: bad step range");
     const Geo g = make_geo(*d);
@@ -644,8 +645,8 @@ extern "C" int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const 
 
 extern "C" int efm_residual(const float *mod, const float *obs, float *res, double *J, long long n,
                             void *stream) {
+    if (n == 0) return EFM_OK;                   // no receivers: nothing to compare
     if (!mod || !obs || !res || !J || n < 0) return fail(EFM_E_ARG, "efm_residual: bad argument");
-    if (n == 0) return EFM_OK;
     hipLaunchKernelGGL(k_residual, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, mod,
                        obs, res, J, n);
     return check_launch("residual");

@@ -112,6 +112,141 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
             }
 }
 
+// Vectorised variant for operands whose contiguous dimension has unit stride
+// (A: k-fast when AK, else m-fast; B: n-fast when BN, else k-fast) and whose
+// other stride is a multiple of 8 elements: each thread moves one 16-byte
+// vector (8 values) of each operand per k-slab, and the next slab is fetched
+// into registers while the matrix cores work on the current one (two LDS
+// stages).  Vectors that cross the M/N/K edge fall back to element loads.
+template <bool AK, bool BN>
+__device__ __forceinline__ void gemm_fetch(uint4 &va, uint4 &vb, int t, int m0, int n0, int k0,
+                                           int M, int N, int K, const uint16_t *__restrict__ A,
+                                           long long lda, const uint16_t *__restrict__ B,
+                                           long long ldb) {
+    // A: AK -> row m = t/4, k = 8(t%4); else k = t/8, m = 8(t%8)
+    {
+        const int r = AK ? (t >> 2) : ((t & 7) << 3), k = AK ? ((t & 3) << 3) : (t >> 3);
+        const int gm = m0 + r, gk = k0 + k;
+        if (AK ? (gm < M && gk + 7 < K) : (gk < K && gm + 7 < M)) {
+            va = *reinterpret_cast<const uint4 *>(AK ? A + (long long)gm * lda + gk
+                                                     : A + (long long)gk * lda + gm);
+        } else {
+            uint16_t e[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int mm = AK ? gm : gm + i, kk = AK ? gk + i : gk;
+                e[i] = (mm < M && kk < K) ? (AK ? A[(long long)mm * lda + kk] : A[(long long)kk * lda + mm])
+                                          : (uint16_t)0;
+            }
+            va = make_uint4(e[0] | (unsigned)e[1] << 16, e[2] | (unsigned)e[3] << 16,
+                            e[4] | (unsigned)e[5] << 16, e[6] | (unsigned)e[7] << 16);
+        }
+    }
+    // B: BN -> k = t/8, n = 8(t%8); else n = t/4, k = 8(t%4)
+    {
+        const int c = BN ? ((t & 7) << 3) : (t >> 2), k = BN ? (t >> 3) : ((t & 3) << 3);
+        const int gn = n0 + c, gk = k0 + k;
+        if (BN ? (gk < K && gn + 7 < N) : (gn < N && gk + 7 < K)) {
+            vb = *reinterpret_cast<const uint4 *>(BN ? B + (long long)gk * ldb + gn
+                                                     : B + (long long)gn * ldb + gk);
+        } else {
+            uint16_t e[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int nn = BN ? gn + i : gn, kk = BN ? gk : gk + i;
+                e[i] = (nn < N && kk < K) ? (BN ? B[(long long)kk * ldb + nn] : B[(long long)nn * ldb + kk])
+                                          : (uint16_t)0;
+            }
+            vb = make_uint4(e[0] | (unsigned)e[1] << 16, e[2] | (unsigned)e[3] << 16,
+                            e[4] | (unsigned)e[5] << 16, e[6] | (unsigned)e[7] << 16);
+        }
+    }
+}
+
+template <bool AK, bool BN>
+__device__ __forceinline__ void gemm_stage(uint16_t *As, uint16_t *Bs, const uint4 &va,
+                                           const uint4 &vb, int t) {
+    if (AK) {
+        *reinterpret_cast<uint4 *>(&As[(t >> 2) * GLD + ((t & 3) << 3)]) = va;
+    } else {
+        const uint16_t *e = reinterpret_cast<const uint16_t *>(&va);
+        const int m = (t & 7) << 3, k = t >> 3;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) As[(m + i) * GLD + k] = e[i];
+    }
+    if (BN) {
+        const uint16_t *e = reinterpret_cast<const uint16_t *>(&vb);
+        const int n = (t & 7) << 3, k = t >> 3;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Bs[(n + i) * GLD + k] = e[i];
+    } else {
+        *reinterpret_cast<uint4 *>(&Bs[(t >> 2) * GLD + ((t & 3) << 3)]) = vb;
+    }
+}
+
+template <bool AK, bool BN>
+__global__ __launch_bounds__(256) void k_gemm_vec(int M, int N, int K,
+                                                  const uint16_t *__restrict__ A, long long lda,
+                                                  const uint16_t *__restrict__ B, long long ldb,
+                                                  float *__restrict__ C, long long ldc,
+                                                  const float *__restrict__ bias,
+                                                  uint16_t *__restrict__ Cb, int epi) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][GBM * GLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][GBN * GLD];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    uint4 va, vb;
+    gemm_fetch<AK, BN>(va, vb, t, m0, n0, 0, M, N, K, A, lda, B, ldb);
+    gemm_stage<AK, BN>(As[0], Bs[0], va, vb, t);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+        const bool more = k0 + GBK < K;
+        if (more) gemm_fetch<AK, BN>(va, vb, t, m0, n0, k0 + GBK, M, N, K, A, lda, B, ldb);
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 16) {
+            bf16x4 a[2], b[2];
+            const int ko = kk + 4 * (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = *reinterpret_cast<const bf16x4 *>(&As[cur][(wm + 16 * i + (lane & 15)) * GLD + ko]);
+                b[i] = *reinterpret_cast<const bf16x4 *>(&Bs[cur][(wn + 16 * i + (lane & 15)) * GLD + ko]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) gemm_stage<AK, BN>(As[cur ^ 1], Bs[cur ^ 1], va, vb, t);
+        __syncthreads();
+        cur ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                const int col = n0 + wn + 16 * j + (lane & 15);
+                if (row < M && col < N) {
+                    float v = acc[i][j][r];
+                    if (epi >= 1) v += bias[row];
+                    if (epi == 1) v = fmaxf(v, 0.f);
+                    const long long o = (long long)row * ldc + col;
+                    C[o] = v;
+                    if (Cb) Cb[o] = f2bf(v);
+                }
+            }
+}
+
 // ---- elementwise / layout kernels -----------------------------------------------
 __global__ void k_cast(const float *__restrict__ x, uint16_t *__restrict__ y, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -319,8 +454,27 @@ extern "C" int efm_net_gemm(int M, int N, int K, const uint16_t *A, long long sa
     NET_CHECK(epilogue == 0 || bias, "efm_net_gemm: bias epilogue needs a bias");
     NET_CHECK((M + GBM - 1) / GBM <= 65535, "efm_net_gemm: M too large");
     dim3 grid((N + GBN - 1) / GBN, (M + GBM - 1) / GBM);
-    k_gemm<<<grid, 256, 0, (hipStream_t)stream>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias,
-                                                  Cb, epilogue);
+    hipStream_t st = (hipStream_t)stream;
+    // vectorised path: unit stride along one dimension of each operand, the
+    // other stride a multiple of 8, 16-byte aligned bases
+    const bool a_ok = ((sak == 1 && sam % 8 == 0) || (sam == 1 && sak % 8 == 0)) &&
+                      ((uintptr_t)A % 16 == 0) && !(sak == 1 && sam == 1 && M > 1 && K > 1);
+    const bool b_ok = ((sbn == 1 && sbk % 8 == 0) || (sbk == 1 && sbn % 8 == 0)) &&
+                      ((uintptr_t)B % 16 == 0) && !(sbn == 1 && sbk == 1 && N > 1 && K > 1);
+    if (a_ok && b_ok) {
+        const bool ak = (sak == 1), bn = (sbn == 1);
+        const long long lda = ak ? sam : sak, ldb = bn ? sbk : sbn;
+        if (ak && bn)
+            k_gemm_vec<true, true><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, Cb, epilogue);
+        else if (ak)
+            k_gemm_vec<true, false><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, Cb, epilogue);
+        else if (bn)
+            k_gemm_vec<false, true><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, Cb, epilogue);
+        else
+            k_gemm_vec<false, false><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, Cb, epilogue);
+        return check_launch("efm_net_gemm");
+    }
+    k_gemm<<<grid, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cb, epilogue);
     return check_launch("efm_net_gemm");
 }
 

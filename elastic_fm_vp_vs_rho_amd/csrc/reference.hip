@@ -143,7 +143,8 @@ static int check_common(const EfmDims *d, const EfmMedium *m, const EfmShots *sh
 
 extern "C" int efm_inject_residual(const EfmDims *d, const EfmShots *sh, const EfmState *st,
                                    const float *res, int it, void *stream) {
-    if (!d || !sh || !st || !res) return fail(EFM_E_ARG, "efm_inject_residual: null argument");
+    if (!d || !sh || !st || (!res && d->nrec > 0))
+        return fail(EFM_E_ARG, "efm_inject_residual: null argument");
     const Geo g = make_geo(*d);
     if (g.nrec <= 0) return EFM_OK;
     hipLaunchKernelGGL(k_inject_ref, dim3(g.S), dim3(64), 0, (hipStream_t)stream, g,
@@ -167,7 +168,8 @@ extern "C" int efm_backward_mt_ref(const EfmDims *d, const EfmMedium *m, const E
                                    void *stream) {
     int rc = check_common(d, m, sh);
     if (rc) return rc;
-    if (!adj || !res || !gm6) return fail(EFM_E_ARG, "efm_backward_mt_ref: null argument");
+    if (!adj || (!res && d->nrec > 0) || !gm6)
+        return fail(EFM_E_ARG, "efm_backward_mt_ref: null argument");
     if (nsteps < 0 || nsteps > d->ntmax) return fail(EFM_E_ARG, "efm_backward_mt_ref: nsteps out of range");
     Geo g = make_geo(*d);
     if (g.c < 0) g.c = -g.c;  // the adjoint runs the forward operator with +dt (M:2066-2068)
@@ -200,7 +202,7 @@ extern "C" int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, con
                                        float *grho, void *stream) {
     int rc = check_common(d, m, sh);
     if (rc) return rc;
-    if (!fwd || !vpres || !borders || !adj || !res || !glam || !gmu || !grho)
+    if (!fwd || !vpres || !borders || !adj || (!res && d->nrec > 0) || !glam || !gmu || !grho)
         return fail(EFM_E_ARG, "efm_backward_remake_ref: null argument");
     if (W + H + 1 > d->nx / 2 || W + H + 1 > d->ny / 2 || W + H + 1 > d->nz / 2)
         return fail(EFM_E_ARG, "efm_backward_remake_ref: CPML too wide for the grid");

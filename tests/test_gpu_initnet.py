@@ -25,11 +25,15 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 32, False, False), (70, 130, 45, False, False),
                                          (9, 5, 300, True, False), (33, 257, 16, False, True),
-                                         (128, 17, 1000, True, True), (1, 1, 1, False, False)])
+                                         (128, 17, 1000, True, True), (1, 1, 1, False, False),
+                                         (100, 72, 40, False, False), (96, 24, 48, True, False),
+                                         (65, 200, 2000, False, False)])
 def test_gemm_against_fp64(gpu_lib, M, N, K, ta, tb):
     """C = A·B for row-major and transposed operand views, ragged edges, and the
     bias/ReLU epilogues (fixes the MFMA lane layout: a transposed or permuted
-    tile fails by O(1))."""
+    tile fails by O(1)).  Shapes whose strides are multiples of 8 take the
+    vectorised kernel (all four operand layouts occur), the others the
+    element-wise one."""
     from elastic_fm_vp_vs_rho_amd.propagator import _ptr, _stream
     from elastic_fm_vp_vs_rho_amd import _capi
     rng = np.random.default_rng(M * 7 + N * 3 + K)

@@ -1,0 +1,51 @@
+"""Host-side argument checks of the engine (CPU only: tensors on the CPU, no
+kernel is launched): a shot batch rejects sources off the interior, receivers
+off the grid and record lengths past the wavelet."""
+import numpy as np
+import pytest
+
+from elastic_fm_vp_vs_rho_amd.propagator import Grid, ShotBatch
+from tests.problems import tiny
+
+
+def _args(p):
+    return dict(src=[p.src], m6=[p.m6], wavelet=[p.wavelet], nt=[p.nt], rec=[p.rec], cpml=p.cp)
+
+
+def _grid(p):
+    W = p.W
+    return Grid(p.nx, p.ny, p.nz, p.dx, p.dt, (W,) * 6)
+
+
+def test_valid_batch_and_linear_receiver_index():
+    p = tiny(n=(20, 18, 16), W=3, seed=1)
+    sh = ShotBatch(_grid(p), device="cpu", **_args(p))
+    x, y, z = p.rec[0]
+    assert int(sh.rec[0, 0]) == x + p.nx * z + p.nx * p.nz * y
+    assert sh.cpml["ax"].shape == (1, p.nx)
+
+
+@pytest.mark.parametrize("src", [(1, 8, 8), (8, 16, 8), (8, 8, 14), (-1, 3, 3)])
+def test_source_off_the_interior_is_rejected(src):
+    p = tiny(n=(20, 18, 16), W=3, seed=1)
+    a = _args(p)
+    a["src"] = [src]
+    with pytest.raises(ValueError, match="interior"):
+        ShotBatch(_grid(p), device="cpu", **a)
+
+
+def test_receiver_off_the_grid_and_long_records_are_rejected():
+    p = tiny(n=(20, 18, 16), W=3, seed=1)
+    a = _args(p)
+    rec = np.array(p.rec).copy()
+    rec[0] = (5, 18, 3)                                   # y = ny
+    a["rec"] = [rec]
+    with pytest.raises(ValueError, match="receiver"):
+        ShotBatch(_grid(p), device="cpu", **a)
+    a = _args(p)
+    a["nt"] = [len(p.wavelet) + 1]
+    with pytest.raises(ValueError, match="record lengths"):
+        ShotBatch(_grid(p), device="cpu", **a)
+    a["nt"] = [0]
+    with pytest.raises(ValueError, match="record lengths"):
+        ShotBatch(_grid(p), device="cpu", **a)
