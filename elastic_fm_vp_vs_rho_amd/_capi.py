@@ -41,7 +41,7 @@ class EfmState(ctypes.Structure):
 
 class EfmShots(ctypes.Structure):
     _fields_ = [(k, c_void_p) for k in ("ax", "bx", "ay", "by", "az", "bz", "src", "m6",
-                                        "wavelet", "nt", "rec")]
+                                        "wavelet", "nt", "rec", "recmask")]
 
 
 P = ctypes.POINTER

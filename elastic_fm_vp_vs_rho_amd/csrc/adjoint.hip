@@ -37,6 +37,8 @@ struct AdjArgs {
     StatePtrs a, a2;               // adjoint: a = live (v̄, τ̄, C in), a2 = C out
     float *acc;                    // (S, 8, N)
     ShotPtrs sh;
+    const float *resid;            // ∂J/∂gathers, used by R4 when the shots carry a receiver mask
+    double *gm6;                   // ∂J/∂M, likewise
 };
 
 // ---------------------------------------------------------------------------
@@ -224,13 +226,37 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_stress(Geo g, AdjArgs A, int
     int x, z, s, yb;
     if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
-    if (n >= A.sh.nt[s]) return;
+    const int nts = A.sh.nt[s];
     const int y0 = H + yb * NYC;
     const int y1 = min(y0 + NYC, g.ny - H);
     const int nx = g.nx, sxz = g.sxz32;
     const long long so = (long long)s * g.N;
     const float c = g.c;
     const ShotPtrs &sh = A.sh;
+    // with a receiver mask this sweep also injects the residual of step n-1 (R1
+    // of the next reverse step) and sums ∂J/∂M of step n (R3)
+    const bool fused = sh.recmask != nullptr;
+    const int *rs = sh.rec + (long long)s * g.nrec;
+    const long long rcs = (long long)g.ntmax * g.nrec;
+    const long long rbase = ((long long)s * 3 * g.ntmax + (n - 1)) * g.nrec;
+    if (n >= nts) {
+        // the shot's record ends at step nts - 1: its first residual is injected here
+        if (fused && n == nts && n >= 1 && g.nrec > 0)
+            for (int y = y0; y < y1; ++y) {
+                const int t = x + z * nx + y * sxz;
+                if (!is_receiver(g, sh, s, t)) continue;
+                for (int r = 0; r < g.nrec; ++r)
+                    if (rs[r] == t) {
+                        A.a.v[0][so + t] += A.resid[rbase + r];
+                        A.a.v[1][so + t] += A.resid[rbase + rcs + r];
+                        A.a.v[2][so + t] += A.resid[rbase + 2 * rcs + r];
+                    }
+            }
+        return;
+    }
+    const int *sp = sh.src + 3 * s;
+    const int tsrc = sp[0] + sp[2] * nx + sp[1] * sxz;
+    const bool inject = fused && n >= 1 && g.nrec > 0;
 
     const int sx0 = slab_coord(x, g.nx, g.wxl, g.wxh);
     const int sxm = slab_coord(x - 1, g.nx, g.wxl, g.wxh);
@@ -371,9 +397,26 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_stress(Geo g, AdjArgs A, int
             Czo[pi] = bz0 * c0; Czo[pi + csz] = bz0 * c1; Czo[pi + 2 * csz] = bz0 * c2;
         }
         // v̄ += D_vᵀ ḡ
-        vb0[t] += (g_vxx_0 - g_vxx_p) + (g_vxz_m - g_vxz_0) + (g_vxy_m - g_vxy_0);
-        vb1[t] += (g_vyy_0 - g_vyy_p) + (g_vyx_m - g_vyx_0) + (g_vyz_m - g_vyz_0);
-        vb2[t] += (g_vzz_0 - g_vzz_p) + (g_vzx_m - g_vzx_0) + (g_vzy_m - g_vzy_0);
+        float nv0 = vb0[t] + ((g_vxx_0 - g_vxx_p) + (g_vxz_m - g_vxz_0) + (g_vxy_m - g_vxy_0));
+        float nv1 = vb1[t] + ((g_vyy_0 - g_vyy_p) + (g_vyx_m - g_vyx_0) + (g_vyz_m - g_vyz_0));
+        float nv2 = vb2[t] + ((g_vzz_0 - g_vzz_p) + (g_vzx_m - g_vzx_0) + (g_vzy_m - g_vzy_0));
+        if (inject && is_receiver(g, sh, s, t)) {      // R1 of step n-1 at this cell's receivers
+            for (int r = 0; r < g.nrec; ++r)
+                if (rs[r] == t) {
+                    nv0 += A.resid[rbase + r];
+                    nv1 += A.resid[rbase + rcs + r];
+                    nv2 += A.resid[rbase + 2 * rcs + r];
+                }
+        }
+        vb0[t] = nv0;
+        vb1[t] = nv1;
+        vb2[t] = nv2;
+        if (fused && t == tsrc) {                       // R3 of step n: ∂J/∂M += w[n]·τ̄(src)
+            const double w = sh.wav[(long long)s * g.ntmax + n];
+            double *gm = A.gm6 + 6 * s;
+            gm[0] += w * (double)txx; gm[1] += w * (double)tyy; gm[2] += w * (double)tzz;
+            gm[3] += w * (double)txy; gm[4] += w * (double)txz; gm[5] += w * (double)tyz;
+        }
 
         // λ, μ accumulators: forward e at p from v^n and ψv^n
         const float v0 = LDF(&V0[t]), v1 = LDF(&V1[t]), v2 = LDF(&V2[t]);
@@ -595,11 +638,23 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
     A.lam = m->lam; A.mu = m->mu; A.mbar = mbar; A.bbar = bbar; A.acc = acc;
     A.sh = make_shots(*sh);
     StatePtrs live = make_state(*adj), other = make_state(*adj2);
-    for (int n = n1 - 1; n >= n0; --n) {
-        hipLaunchKernelGGL(k_adj_points, dim3(g.S), dim3(64), 0, st, g, A.sh, live, resid, gm6, n,
-                           n + 1 < d->ntmax ? n + 1 : -1);
+    A.resid = resid;
+    A.gm6 = gm6;
+    const bool fused = A.sh.recmask != nullptr;
+    if (fused && n1 == d->ntmax) {
+        // first residual injection of the run; the sweeps do the rest
+        hipLaunchKernelGGL(k_adj_points, dim3(g.S), dim3(64), 0, st, g, A.sh, live, resid, gm6,
+                           n1 - 1, -1);
         int rc = check_launch("adjoint points");
         if (rc) return rc;
+    }
+    for (int n = n1 - 1; n >= n0; --n) {
+        int rc = EFM_OK;
+        if (!fused) {
+            hipLaunchKernelGGL(k_adj_points, dim3(g.S), dim3(64), 0, st, g, A.sh, live, resid, gm6,
+                               n, n + 1 < d->ntmax ? n + 1 : -1);
+            if ((rc = check_launch("adjoint points"))) return rc;
+        }
         A.fn = make_state(*fwd[n - n0]);
         A.fn1 = make_state(*fwd[n - n0 + 1]);
         A.a = live;
@@ -612,7 +667,7 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
         if ((rc = launch_adj_stress(g, A, n, st))) return rc;
         std::swap(live.px, other.px); std::swap(live.py, other.py); std::swap(live.pz, other.pz);
     }
-    if (last && n0 == 0) {
+    if (!fused && last && n0 == 0) {
         hipLaunchKernelGGL(k_adj_points, dim3(g.S), dim3(64), 0, st, g, A.sh, live, resid, gm6, -1, 0);
         int rc = check_launch("adjoint points");
         if (rc) return rc;

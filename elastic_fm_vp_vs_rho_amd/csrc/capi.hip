@@ -151,8 +151,8 @@ extern "C" int efm_forward(const EfmDims *d, const EfmMedium *m, const EfmShots 
     hipStream_t hs = (hipStream_t)stream;
     for (int it = it0; it < it0 + nsteps; ++it) {
         if ((rc = launch_stress_src(g, *m, shp, s, s, it, 1, hs))) return rc;
-        if ((rc = launch_velocity(g, *m, shp, s, s, it, hs))) return rc;
-        if (gathers && (rc = launch_record(g, shp, s, gathers, it, hs))) return rc;
+        if ((rc = launch_velocity(g, *m, shp, s, s, it, hs, gathers))) return rc;
+        if (gathers && !shp.recmask && (rc = launch_record(g, shp, s, gathers, it, hs))) return rc;
         if (borders && (rc = launch_border(g, shp, s, borders, W, it, 1, it, hs))) return rc;
     }
     return EFM_OK;
@@ -190,8 +190,8 @@ extern "C" int efm_forward_graph(const EfmDims *d, const EfmMedium *m, const Efm
         int crc = EFM_OK;
         for (int k = 0; k < chunk && crc == EFM_OK; ++k) {
             if ((crc = launch_stress_src(g, *m, shp, s, s, k, 1, hs))) break;
-            if ((crc = launch_velocity(g, *m, shp, s, s, k, hs))) break;
-            if (gathers) crc = launch_record(g, shp, s, gathers, k, hs);
+            if ((crc = launch_velocity(g, *m, shp, s, s, k, hs, gathers))) break;
+            if (gathers && !shp.recmask) crc = launch_record(g, shp, s, gathers, k, hs);
         }
         if (crc == EFM_OK) {
             hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, hs, counter, chunk);
@@ -216,8 +216,8 @@ extern "C" int efm_forward_graph(const EfmDims *d, const EfmMedium *m, const Efm
     const Geo g = make_geo(*d);
     for (int it = it0 + nchunks * chunk; it < it0 + nsteps; ++it) {
         if ((rc = launch_stress_src(g, *m, shp, s, s, it, 1, hs))) return rc;
-        if ((rc = launch_velocity(g, *m, shp, s, s, it, hs))) return rc;
-        if (gathers && (rc = launch_record(g, shp, s, gathers, it, hs))) return rc;
+        if ((rc = launch_velocity(g, *m, shp, s, s, it, hs, gathers))) return rc;
+        if (gathers && !shp.recmask && (rc = launch_record(g, shp, s, gathers, it, hs))) return rc;
     }
     return EFM_OK;
 }
@@ -238,8 +238,9 @@ extern "C" int efm_forward_states(const EfmDims *d, const EfmMedium *m, const Ef
     for (int k = 0; k < nsteps; ++k) {
         const StatePtrs in = make_state(*states[k]), out = make_state(*states[k + 1]);
         if ((rc = launch_stress_src(g, *m, shp, in, out, it0 + k, 1, hs))) return rc;
-        if ((rc = launch_velocity(g, *m, shp, in, out, it0 + k, hs))) return rc;
-        if (gathers && (rc = launch_record(g, shp, out, gathers, it0 + k, hs))) return rc;
+        if ((rc = launch_velocity(g, *m, shp, in, out, it0 + k, hs, gathers))) return rc;
+        if (gathers && !shp.recmask && (rc = launch_record(g, shp, out, gathers, it0 + k, hs)))
+            return rc;
     }
     return EFM_OK;
 }

@@ -25,6 +25,7 @@ struct Geo {
     int sxz32;                         // nx*nz as int (per-shot offsets are 32-bit)
     int pxc, pyc, pzc;                 // per-component strides of the ψ slabs
     const int *itp;                    // device time-step counter added to `it` (graph replay), or null
+    int recw;                          // words per shot of the receiver bitmask, ceil(N/32)
 };
 
 struct StatePtrs {
@@ -40,6 +41,7 @@ struct ShotPtrs {
     const float *wav;
     const int *nt;
     const int *rec;
+    const unsigned *recmask;   // (S, recw) receiver bitmask, or null
 };
 
 inline Geo make_geo(const EfmDims &d) {
@@ -57,6 +59,7 @@ inline Geo make_geo(const EfmDims &d) {
     g.pyc = g.Ly * d.nz * d.nx;
     g.pzc = d.ny * g.Lz * d.nx;
     g.itp = nullptr;
+    g.recw = (int)((g.N + 31) / 32);
     return g;
 }
 
@@ -73,6 +76,7 @@ inline ShotPtrs make_shots(const EfmShots &s) {
     ShotPtrs p;
     p.ax = s.ax; p.bx = s.bx; p.ay = s.ay; p.by = s.by; p.az = s.az; p.bz = s.bz;
     p.src = s.src; p.m6 = s.m6; p.wav = s.wavelet; p.nt = s.nt; p.rec = s.rec;
+    p.recmask = s.recmask;
     return p;
 }
 
@@ -108,6 +112,11 @@ __device__ __forceinline__ long long pz_index(const Geo &g, int s, int comp, int
 __device__ __forceinline__ long long pz_cstride(const Geo &g) { return (long long)g.ny * g.Lz * g.nx; }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Whether cell t (per-shot linear index) of shot s holds a receiver.
+__device__ __forceinline__ bool is_receiver(const Geo &g, const ShotPtrs &sh, int s, int t) {
+    return (sh.recmask[(long long)s * g.recw + (t >> 5)] >> (t & 31)) & 1u;
+}
 
 __device__ __forceinline__ bool interior(int x, int y, int z, const Geo &g) {
     return x >= H && x < g.nx - H && y >= H && y < g.ny - H && z >= H && z < g.nz - H;
@@ -208,7 +217,7 @@ int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, cons
 int launch_stress(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
                   const StatePtrs &out, int it, hipStream_t st);
 int launch_velocity(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
-                    const StatePtrs &out, int it, hipStream_t st);
+                    const StatePtrs &out, int it, hipStream_t st, float *gathers = nullptr);
 int launch_record(const Geo &g, const ShotPtrs &sh, const StatePtrs &s, float *gathers, int it,
                   hipStream_t st);
 int launch_border(const Geo &g, const ShotPtrs &sh, const StatePtrs &s, float *borders, int W,

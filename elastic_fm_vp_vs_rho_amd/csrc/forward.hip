@@ -145,7 +145,7 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_stress(Geo g, Med m, ShotPtrs sh
 // ---------------------------------------------------------------------------
 template <int BX, int BZ, int NYC, int MAP>
 __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_velocity(Geo g, Med m, ShotPtrs sh, StatePtrs in,
-                                                         StatePtrs out, int it) {
+                                                         StatePtrs out, int it, float *gathers) {
     if (g.itp) it += *g.itp;
     int x, z, s, yb;
     if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
@@ -223,9 +223,24 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_velocity(Geo g, Med m, ShotPtrs 
         const float bbx = 2.f * frcp(r + rho[t + 1]);
         const float bby = 2.f * frcp(r + rho[t + sxz]);
         const float bbz = 2.f * frcp(r + rho[t + nx]);
-        vo0[t] = vi0[t] + c * bbx * (fxx + fxy + fxz);
-        vo1[t] = vi1[t] + c * bby * (fyx + fyy + fyz);
-        vo2[t] = vi2[t] + c * bbz * (fzx + fzy + fzz);
+        const float n0 = vi0[t] + c * bbx * (fxx + fxy + fxz);
+        const float n1 = vi1[t] + c * bby * (fyx + fyy + fyz);
+        const float n2 = vi2[t] + c * bbz * (fzx + fzy + fzz);
+        vo0[t] = n0;
+        vo1[t] = n1;
+        vo2[t] = n2;
+        // receiver sampling by the cell's own thread (kernel_get_Station, M:324-335)
+        if (gathers && is_receiver(g, sh, s, t)) {
+            const int *rs = sh.rec + (long long)s * g.nrec;
+            const long long cs = (long long)g.ntmax * g.nrec;
+            const long long base = ((long long)s * 3 * g.ntmax + it) * g.nrec;
+            for (int r = 0; r < g.nrec; ++r)
+                if (rs[r] == t) {
+                    gathers[base + r] = n0;
+                    gathers[base + cs + r] = n1;
+                    gathers[base + 2 * cs + r] = n2;
+                }
+        }
     }
 }
 
@@ -325,11 +340,13 @@ int launch_stress(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const St
 }
 
 int launch_velocity(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
-                    const StatePtrs &out, int it, hipStream_t st) {
+                    const StatePtrs &out, int it, hipStream_t st, float *gathers) {
     const Tile t = get_tile(1);
     Med md{m.lam, m.mu, m.rho};
     bool launched = false;
-#define X(a, b, c, o) EFM_LAUNCH_IF(a, b, c, o, k_velocity, g, md, sh, in, out, it)
+    // the sweep samples the receivers only when the shots carry a receiver bitmask
+    float *gat = (sh.recmask && g.nrec > 0) ? gathers : nullptr;
+#define X(a, b, c, o) EFM_LAUNCH_IF(a, b, c, o, k_velocity, g, md, sh, in, out, it, gat)
     EFM_TILE_LIST(X)
 #undef X
     if (!launched) return fail(EFM_E_UNSUPPORTED, "unsupported velocity tile");

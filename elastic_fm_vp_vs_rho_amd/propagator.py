@@ -70,7 +70,7 @@ class ShotBatch:
     (S, ntmax); nt (S,); rec (S, nrec, 3) integer (x, y, z), -1 rows unused;
     cpml: dict of (S, n) arrays ax, bx, ay, by, az, bz."""
 
-    def __init__(self, grid: Grid, src, m6, wavelet, nt, rec, cpml, device="cuda"):
+    def __init__(self, grid: Grid, src, m6, wavelet, nt, rec, cpml, device="cuda", fuse_receivers=True):
         self.grid = grid
         src = np.asarray(src, np.int64).reshape(-1, 3)
         S = src.shape[0]
@@ -95,6 +95,16 @@ class ShotBatch:
         self.src_host, self.nt_host, self.rec_host = src, nt, rec
         self.src, self.m6, self.wavelet = i32(src), f32(np.asarray(m6).reshape(S, 6)), f32(wavelet)
         self.nt, self.rec = i32(nt), i32(lin)
+        # receiver bitmask (S, ceil(N/32)): the sweeps sample receivers and inject
+        # residuals at their own cells instead of separate kernels each step
+        self.recmask = None
+        if fuse_receivers:
+            words = (grid.N + 31) // 32
+            mask = np.zeros((S, words), np.uint32)
+            for s in range(S):
+                cells = lin[s][lin[s] >= 0]
+                np.bitwise_or.at(mask[s], cells // 32, (np.uint32(1) << (cells % 32).astype(np.uint32)))
+            self.recmask = torch.as_tensor(mask.view(np.int32), device=dev)
         prof = {}
         for k, n in (("ax", grid.nx), ("bx", grid.nx), ("ay", grid.ny), ("by", grid.ny),
                      ("az", grid.nz), ("bz", grid.nz)):
@@ -109,7 +119,7 @@ class ShotBatch:
         c = self.cpml
         return EfmShots(_ptr(c["ax"]), _ptr(c["bx"]), _ptr(c["ay"]), _ptr(c["by"]), _ptr(c["az"]),
                         _ptr(c["bz"]), _ptr(self.src), _ptr(self.m6), _ptr(self.wavelet),
-                        _ptr(self.nt), _ptr(self.rec))
+                        _ptr(self.nt), _ptr(self.rec), _ptr(self.recmask))
 
 
 def component_pad():

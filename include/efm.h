@@ -69,6 +69,11 @@ typedef struct EfmShots {      /* per-shot description */
     const float *wavelet;      /* (S, ntmax) */
     const int   *nt;           /* (S) record length of each shot */
     const int   *rec;          /* (S, nrec) */
+    /* (S, ceil(N/32)) bit set for every cell that holds a receiver of the shot
+     * (bit c%32 of word c/32), or null.  When given, the velocity sweep samples
+     * the receivers itself and the adjoint stress sweep injects the residual
+     * and sums ∂J/∂M, instead of separate small kernels each step. */
+    const unsigned *recmask;
 } EfmShots;
 
 /* ---- library ---------------------------------------------------------- */
@@ -152,8 +157,12 @@ int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float 
  * adj/adj2 are the adjoint state and a second ψ buffer (ping-pong; on return
  * the live ψ adjoint is in adj).  resid is ∂J/∂gathers (S, 3, ntmax, nrec).
  * acc (S·8 components of stride N + EFM_PAD) accumulates λ, μ_n, μ_xy, μ_xz,
- * μ_yz, ρx, ρy, ρz sums; gm6 (S, 6) double accumulates ∂J/∂M.  `last` = 1 when n0 is the
- * first step of the run (closes the moment-tensor sum of step n0). */
+ * μ_yz, ρx, ρy, ρz sums; gm6 (S, 6) double accumulates ∂J/∂M.  The run is
+ * covered by calls over consecutive descending ranges, the first with
+ * n1 = ntmax.  Without a receiver mask `last` = 1 marks the call with n0 = 0
+ * (it closes the moment-tensor sum of step 0); with one, the adjoint stress
+ * sweep injects each next step's residual and sums ∂J/∂M itself and `last`
+ * is not used. */
 int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mbar, const float *bbar,
                 const EfmShots *sh, const EfmState *const *fwd, const EfmState *adj,
                 const EfmState *adj2, const float *resid, float *acc, double *gm6,
