@@ -90,8 +90,10 @@ class ShotBatch:
         lin = np.where(used, rx + grid.nx * rz + grid.nx * grid.nz * ry, -1)
         self.S, self.ntmax, self.nrec = S, wavelet.shape[1], rec.shape[1]
         dev = torch.device(device)
-        f32 = lambda a: torch.as_tensor(np.array(a, np.float32), device=dev)
-        i32 = lambda a: torch.as_tensor(np.array(a, np.int32), device=dev)
+        # C-ordered, writable copies: the kernels index (S, n) arrays row-major
+        # (a broadcast view copied in its own memory order would be column-major)
+        f32 = lambda a: torch.as_tensor(np.array(a, np.float32, order="C"), device=dev)
+        i32 = lambda a: torch.as_tensor(np.array(a, np.int32, order="C"), device=dev)
         self.src_host, self.nt_host, self.rec_host = src, nt, rec
         self.src, self.m6, self.wavelet = i32(src), f32(np.asarray(m6).reshape(S, 6)), f32(wavelet)
         self.nt, self.rec = i32(nt), i32(lin)
@@ -104,13 +106,15 @@ class ShotBatch:
             for s in range(S):
                 cells = lin[s][lin[s] >= 0]
                 np.bitwise_or.at(mask[s], cells // 32, (np.uint32(1) << (cells % 32).astype(np.uint32)))
-            self.recmask = torch.as_tensor(mask.view(np.int32), device=dev)
+            self.recmask = i32(mask.view(np.int32))
         prof = {}
         for k, n in (("ax", grid.nx), ("bx", grid.nx), ("ay", grid.ny), ("by", grid.ny),
                      ("az", grid.nz), ("bz", grid.nz)):
             a = np.asarray(cpml[k], np.float32)
             prof[k] = f32(np.broadcast_to(a, (S, n)))
         self.cpml = prof
+        for t in (self.src, self.m6, self.wavelet, self.nt, self.rec, *prof.values()):
+            assert t.is_contiguous()
 
     def set_m6(self, m6):
         self.m6.copy_(torch.as_tensor(np.asarray(m6, np.float32).reshape(self.S, 6)))

@@ -49,3 +49,17 @@ def test_receiver_off_the_grid_and_long_records_are_rejected():
     a["nt"] = [0]
     with pytest.raises(ValueError, match="record lengths"):
         ShotBatch(_grid(p), device="cpu", **a)
+
+
+def test_batch_arrays_are_row_major_per_shot():
+    """Per-shot arrays reach the kernels row-major (S, n): shot s's CPML profile
+    starts at s*n, also when one profile is broadcast to every shot."""
+    p = tiny(n=(20, 18, 16), W=3, seed=1)
+    g = _grid(p)
+    sh = ShotBatch(g, [p.src] * 3, [p.m6] * 3, [p.wavelet] * 3, [p.nt] * 3, [p.rec] * 3, p.cp, device="cpu")
+    for k, t in sh.cpml.items():
+        assert t.is_contiguous() and t.stride() == (t.shape[1], 1), k
+        flat = t.reshape(-1).numpy()
+        n = t.shape[1]
+        for s in range(3):
+            np.testing.assert_array_equal(flat[s * n:(s + 1) * n], np.asarray(p.cp[k], np.float32))
