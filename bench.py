@@ -40,28 +40,28 @@ ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 12
 # the 128^3 grid; any other size reports traffic null.
 MEASURED_TRAFFIC = {
     "stress": {
-        "64,4,1,0": 177758600, "64,1,1,0": 208423755, "64,2,1,0": 187290808, "128,1,1,0": 191204195,
-        "128,2,1,0": 171726402, "64,4,2,0": 183123444, "64,1,1,8": 155798283, "64,2,1,8": 155796939,
-        "64,4,1,8": 155802026, "64,8,1,8": 155798123, "128,1,1,8": 155804650, "128,2,1,8": 155799915,
-        "256,1,1,8": 155788332, "64,4,2,8": 156076752,
+        "64,4,1,0": 177754539, "64,1,1,0": 208426641, "64,2,1,0": 187283109, "128,1,1,0": 191200557,
+        "128,2,1,0": 171719701, "64,4,2,0": 182638955, "64,1,1,8": 155795491, "64,2,1,8": 155797538,
+        "64,4,1,8": 155788675, "64,8,1,8": 155796418, "128,1,1,8": 155797474, "128,2,1,8": 155796002,
+        "256,1,1,8": 155796226, "64,4,2,8": 155952369,
     },
     "velocity": {
-        "64,4,1,0": 145703531, "64,1,1,0": 174883745, "64,2,1,0": 154744812, "128,1,1,0": 159619191,
-        "128,2,1,0": 140187058, "64,4,2,0": 154390543, "64,1,1,8": 124288216, "64,2,1,8": 124294520,
-        "64,4,1,8": 124292600, "64,8,1,8": 124284921, "128,1,1,8": 124295160, "128,2,1,8": 124298039,
-        "256,1,1,8": 124279513, "64,4,2,8": 125238427,
+        "64,4,1,0": 146728292, "64,1,1,0": 174895585, "64,2,1,0": 154757509, "128,1,1,0": 159644256,
+        "128,2,1,0": 140191749, "64,4,2,0": 148158085, "64,1,1,8": 124295153, "64,2,1,8": 124306735,
+        "64,4,1,8": 124307695, "64,8,1,8": 124556884, "128,1,1,8": 124296848, "128,2,1,8": 124304752,
+        "256,1,1,8": 124298096, "64,4,2,8": 124420835,
     },
     "adj_velocity": {
-        "64,4,1,0": 327312657, "64,1,1,0": 391132481, "64,2,1,0": 348036013, "128,1,1,0": 362313621,
-        "128,2,1,0": 318771747, "64,4,2,0": 374315746, "64,1,1,8": 298119073, "64,2,1,8": 297790657,
-        "64,4,1,8": 297088519, "64,8,1,8": 296897178, "128,1,1,8": 298098627, "128,2,1,8": 296407978,
-        "256,1,1,8": 298547286, "64,4,2,8": 316725903,
+        "64,4,1,0": 327631894, "64,1,1,0": 391259077, "64,2,1,0": 347957214, "128,1,1,0": 362624023,
+        "128,2,1,0": 318753629, "64,4,2,0": 373751556, "64,1,1,8": 297847223, "64,2,1,8": 297807035,
+        "64,4,1,8": 296667290, "64,8,1,8": 296892641, "128,1,1,8": 298028418, "128,2,1,8": 296165906,
+        "256,1,1,8": 298346943, "64,4,2,8": 317227304,
     },
     "adj_stress": {
-        "64,4,1,0": 338542509, "64,1,1,0": 421641960, "64,2,1,0": 365565849, "128,1,1,0": 386331231,
-        "128,2,1,0": 329850287, "64,4,2,0": 407310196, "64,1,1,8": 300014650, "64,2,1,8": 299700921,
-        "64,4,1,8": 298999615, "64,8,1,8": 299529355, "128,1,1,8": 300260642, "128,2,1,8": 298463155,
-        "256,1,1,8": 299857129, "64,4,2,8": 328074528,
+        "64,4,1,0": 338586213, "64,1,1,0": 423083454, "64,2,1,0": 366761377, "128,1,1,0": 387776632,
+        "128,2,1,0": 329903669, "64,4,2,0": 408266097, "64,1,1,8": 299929410, "64,2,1,8": 299599687,
+        "64,4,1,8": 298649170, "64,8,1,8": 299208883, "128,1,1,8": 300253119, "128,2,1,8": 298024758,
+        "256,1,1,8": 299641922, "64,4,2,8": 330245842,
     },
 }
 
