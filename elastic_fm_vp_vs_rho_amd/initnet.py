@@ -103,18 +103,33 @@ class InitNet:
         self.dev = torch.device(device)
         self.L = _capi.lib()
         p = he_init(spec, seed) if params is None else params
+        # every parameter lives in one flat buffer (offsets aligned to 64 values,
+        # so each view keeps 16-byte alignment for the vector loads), and so do its
+        # BF16 copy, gradient and Adam moments: one Adam launch updates them all
+        shapes = spec.shapes()
+        offs, n = {}, 0
+        for k, shp in shapes.items():
+            offs[k] = n
+            n += (int(np.prod(shp)) + 63) // 64 * 64
+        self.nflat = n
+        self.flat_w = torch.zeros(n, device=self.dev)
+        self.flat_wb = torch.zeros(n, dtype=torch.bfloat16, device=self.dev)
+        self.flat_g = torch.zeros(n, device=self.dev)
+        self.flat_m = torch.zeros(n, device=self.dev)
+        self.flat_v = torch.zeros(n, device=self.dev)
+        view = lambda flat, k: flat[offs[k]:offs[k] + int(np.prod(shapes[k]))].view(shapes[k])
         self.w, self.wb, self.g, self.m, self.v = {}, {}, {}, {}, {}
-        for k, shp in spec.shapes().items():
+        for k, shp in shapes.items():
             a = np.asarray(p[k], np.float32)
             if a.shape != tuple(shp):
                 raise ValueError(f"parameter {k}: shape {a.shape}, expected {shp}")
-            self.w[k] = torch.as_tensor(a, device=self.dev).contiguous()
-            self.wb[k] = _bf(shp, self.dev)
-            self.g[k] = torch.zeros(shp, device=self.dev)
-            self.m[k] = torch.zeros(shp, device=self.dev)
-            self.v[k] = torch.zeros(shp, device=self.dev)
+            self.w[k] = view(self.flat_w, k)
+            self.w[k].copy_(torch.as_tensor(a))
+            self.wb[k], self.g[k] = view(self.flat_wb, k), view(self.flat_g, k)
+            self.m[k], self.v[k] = view(self.flat_m, k), view(self.flat_v, k)
         self.t = 0
         self._bufs = {}
+        self._graphs = {}
         self.sync_bf16()
 
     # ---- helpers -------------------------------------------------------------
@@ -122,9 +137,8 @@ class InitNet:
         _capi.check(rc, what)
 
     def sync_bf16(self):
-        for k, w in self.w.items():
-            self._c(self.L.efm_net_cast_bf16(_ptr(w), _ptr(self.wb[k]), w.numel(), _stream()),
-                    "efm_net_cast_bf16")
+        self._c(self.L.efm_net_cast_bf16(_ptr(self.flat_w), _ptr(self.flat_wb), self.nflat, _stream()),
+                "efm_net_cast_bf16")
 
     def params_numpy(self):
         return {k: w.detach().cpu().numpy().copy() for k, w in self.w.items()}
@@ -227,12 +241,12 @@ class InitNet:
         return b["loss"]
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        """One Adam step over every parameter (the alignment gaps hold zeros and stay zero)."""
         self.t += 1
         c1, c2 = 1.0 / (1 - b1 ** self.t), 1.0 / (1 - b2 ** self.t)
-        for k, w in self.w.items():
-            self._c(self.L.efm_net_adam(_ptr(w), _ptr(self.g[k]), _ptr(self.m[k]), _ptr(self.v[k]),
-                                        _ptr(self.wb[k]), w.numel(), lr, b1, b2, eps, c1, c2,
-                                        _stream()), "efm_net_adam")
+        self._c(self.L.efm_net_adam(_ptr(self.flat_w), _ptr(self.flat_g), _ptr(self.flat_m),
+                                    _ptr(self.flat_v), _ptr(self.flat_wb), self.nflat, lr, b1, b2, eps,
+                                    c1, c2, _stream()), "efm_net_adam")
 
     # ---- data ------------------------------------------------------------------
     def load_batch(self, data, targets, idx):
@@ -256,10 +270,35 @@ class InitNet:
         self._buffers(B)["x"].copy_(x)
         return B
 
-    def train_step(self, data, targets, idx, lr=1e-3):
-        B = self.load_batch(data, targets, idx)
-        self.forward_buffers(B)
-        loss = self.backward_buffers(B)
+    def train_step(self, data, targets, idx, lr=1e-3, graph=False):
+        """One Adam step on the mini-batch `idx`.  graph=True replays a hipGraph of
+        the batch assembly, forward and backward passes (captured on first use for
+        this dataset and batch size; the index list is copied into the graph's
+        own buffer), then launches Adam."""
+        if not graph:
+            B = self.load_batch(data, targets, idx)
+            self.forward_buffers(B)
+            loss = self.backward_buffers(B)
+            self.adam(lr)
+            return loss
+        B = int(idx.numel())
+        key = (B, data.data_ptr(), targets.data_ptr())
+        gr = self._graphs.get(key)
+        if gr is None:
+            static_idx = idx.clone()
+            self.load_batch(data, targets, static_idx)          # eager warm-up: buffers exist
+            self.forward_buffers(B)
+            self.backward_buffers(B)
+            torch.cuda.synchronize(self.dev)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.load_batch(data, targets, static_idx)
+                self.forward_buffers(B)
+                loss = self.backward_buffers(B)
+            gr = self._graphs[key] = (g, static_idx, loss)
+        g, static_idx, loss = gr
+        static_idx.copy_(idx)
+        g.replay()
         self.adam(lr)
         return loss
 

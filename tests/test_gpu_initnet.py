@@ -127,6 +127,41 @@ def test_forward_backward_against_oracle(gpu_lib):
         np.testing.assert_array_equal(net.wb[k].float().cpu().numpy(), R.bf16(net.w[k].cpu().numpy()))
 
 
+def test_graph_training_step_equals_eager(gpu_lib):
+    """Training steps replayed from a captured hipGraph (batch assembly, forward,
+    backward) give the same weights, bit for bit, as the same steps launched
+    eagerly, over several replays with eager work in between that overwrites the
+    graph's own buffers (a prediction at the same batch size), and for two batch
+    sizes each with its own graph.  The loss (an atomic FP64 sum) agrees to
+    1e-12."""
+    from elastic_fm_vp_vs_rho_amd.initnet import InitNet, NetSpec
+    spec = NetSpec(**SMALL)
+    rng = np.random.default_rng(21)
+    N = 40
+    data = _bf(rng.standard_normal((N, spec.cin, spec.length)))
+    tg = torch.as_tensor(rng.standard_normal((N, 9)).astype(np.float32), device="cuda")
+    eager = InitNet(spec, seed=4, device="cuda")
+    graph = InitNet(spec, seed=4, device="cuda")
+    assert torch.equal(eager.flat_w, graph.flat_w)
+    steps = [(6, 0), (6, 1), (4, 2), (6, 3), (4, 4), (6, 5), (6, 6)]
+    for B, seed in steps:
+        idx = torch.as_tensor(np.random.default_rng(seed).choice(N, B, replace=False).astype(np.int32),
+                              device="cuda")
+        le = eager.train_step(data, tg, idx, lr=1e-3)
+        lg = graph.train_step(data, tg, idx, lr=1e-3, graph=True)
+        torch.cuda.synchronize()
+        assert abs(float(lg) - float(le)) <= 1e-12 * abs(float(le)), (B, seed)
+        assert torch.equal(graph.flat_w, eager.flat_w), (B, seed)
+        assert torch.equal(graph.flat_wb, eager.flat_wb), (B, seed)
+        # eager work between replays, on the graph's own B-sized buffers
+        p_g = graph.predict(data, idx[:B].flip(0), batch=B)
+        p_e = eager.predict(data, idx[:B].flip(0), batch=B)
+        torch.cuda.synchronize()
+        assert torch.equal(p_g, p_e)
+    assert len(graph._graphs) == 2
+    assert graph.t == eager.t == len(steps)
+
+
 def test_pack_and_gather_batch(gpu_lib):
     from elastic_fm_vp_vs_rho_amd.initnet import InitNet, NetSpec, pack_gathers
     rng = np.random.default_rng(5)
