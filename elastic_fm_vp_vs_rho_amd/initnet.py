@@ -282,7 +282,7 @@ class InitNet:
             self.adam(lr)
             return loss
         B = int(idx.numel())
-        key = (B, data.data_ptr(), targets.data_ptr())
+        key = (B, data.data_ptr(), tuple(data.shape), targets.data_ptr(), tuple(targets.shape))
         gr = self._graphs.get(key)
         if gr is None:
             static_idx = idx.clone()
