@@ -31,17 +31,26 @@ class ElasticModeling(torch.autograd.Function):
         tape = prop.record(K, strategy="checkpoint" if K is not None else "auto")
         ctx.prop, ctx.tape = prop, tape
         ctx.save_for_backward(vp, vs, rho)
-        ctx.m6_dtype = m6.dtype
+        ctx.m6_dtype, ctx.m6_shape = m6.dtype, m6.shape
+        # the backward may re-run forward segments from checkpoints: it needs the
+        # moment tensors of this call even if the batch is reused meanwhile
+        ctx.m6_used = shots.m6.clone()
         return tape["gathers"]
 
     @staticmethod
     def backward(ctx, grad_g):
         vp, vs, rho = ctx.saved_tensors
-        glam, gmu, grho, gm6 = ctx.prop.backprop(ctx.tape, grad_g)
+        shots = ctx.prop.shots
+        m6_now = shots.m6.clone()
+        shots.m6.copy_(ctx.m6_used)
+        try:
+            glam, gmu, grho, gm6 = ctx.prop.backprop(ctx.tape, grad_g)
+        finally:
+            shots.m6.copy_(m6_now)
         ctx.tape = None
         ctx.prop.release_states()
         gvp, gvs, grho_total = lame_adjoint(vp, vs, rho, glam, gmu, grho)
-        return gvp, gvs, grho_total, gm6.to(ctx.m6_dtype), None, None, None
+        return (gvp, gvs, grho_total, gm6.to(ctx.m6_dtype).reshape(ctx.m6_shape), None, None, None)
 
 
 def elastic_modeling(vp, vs, rho, m6, grid: Grid, shots: ShotBatch, K=None):

@@ -38,6 +38,38 @@ def test_autograd_matches_oracle_velocity_gradients(gpu_lib):
     assert rel_l2(m6.grad[0].double().cpu().numpy(), ga["m6"]) < 1e-4
 
 
+def test_autograd_reused_shot_batch_and_flat_m6(gpu_lib):
+    """Two differentiable calls on one shot batch with different moment tensors
+    (checkpointed, so the backward re-runs forward segments after the second
+    call changed the batch's M) give the gradients of two separate batches; a
+    (6,) moment tensor gets a (6,) gradient."""
+    from elastic_fm_vp_vs_rho_amd.ops import elastic_modeling
+    p = tiny(n=(26, 20, 18), nt=70, W=3, seed=22, nrec=5)
+    grid = grid_of(p)
+    dev = "cuda"
+    obs = torch.tensor(run_forward(p, lam=p.lam * 1.05)[0].astype(np.float32), device=dev)
+    ma = np.asarray(p.m6, np.float32)
+    mb = np.roll(ma, 2) * np.float32(-0.7)
+
+    def grads(shared):
+        vp = torch.tensor(p.vp, device=dev, requires_grad=True)
+        vs = torch.tensor(p.vs, device=dev, requires_grad=True)
+        rho = torch.tensor(p.rho, device=dev, requires_grad=True)
+        m1 = torch.tensor(ma, device=dev, requires_grad=True)
+        m2 = torch.tensor(mb, device=dev, requires_grad=True)
+        s1 = shots_of(p, grid)
+        s2 = s1 if shared else shots_of(p, grid)
+        g1 = elastic_modeling(vp, vs, rho, m1, grid, s1, K=8)
+        g2 = elastic_modeling(vp, vs, rho, m2, grid, s2, K=8)
+        loss = 0.5 * ((g1[0] - obs) ** 2).sum() + 0.5 * ((g2[0] - obs) ** 2).sum()
+        loss.backward()
+        assert m1.grad.shape == (6,)
+        return [t.grad.double() for t in (vp, vs, rho, m1, m2)]
+
+    for a, b in zip(grads(True), grads(False)):
+        assert float((a - b).abs().max()) <= 1e-6 * float(b.abs().max())
+
+
 def _smooth_probe(shape, gen, W, dev):
     d = torch.randn((1, 1) + shape, device=dev, generator=gen)
     k = torch.exp(-0.5 * (torch.arange(-6, 7, device=dev, dtype=torch.float32) / 3.0) ** 2)
