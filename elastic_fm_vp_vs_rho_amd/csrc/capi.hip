@@ -24,9 +24,12 @@ int check_launch(const char *what) {
     return EFM_OK;
 }
 
-// Tiles: 0 stress, 1 velocity, 2 adjoint velocity, 3 adjoint stress.
+// Tiles: 0 stress, 1 velocity, 2 adjoint velocity, 3 adjoint stress.  The
+// default, 128 x 1 threads in the XCD-aware order, measured within a few per
+// cent of the autotuner's choice for every kernel at 128^3, 256x256x128 and
+// 200x200x100.
 static std::mutex g_tile_mu;
-static Tile g_tiles[4] = {{64, 4, 1, 0}, {64, 4, 1, 0}, {64, 4, 1, 0}, {64, 4, 1, 0}};
+static Tile g_tiles[4] = {{128, 1, 1, 8}, {128, 1, 1, 8}, {128, 1, 1, 8}, {128, 1, 1, 8}};
 #define EFM_TILE_ENTRY(a, b, c, o) {a, b, c, o},
 static const Tile kTiles[] = {EFM_TILE_LIST(EFM_TILE_ENTRY)};
 #undef EFM_TILE_ENTRY

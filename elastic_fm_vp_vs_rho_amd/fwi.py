@@ -21,7 +21,7 @@ import time
 import numpy as np
 import torch
 
-from . import lbfgs, workloads
+from . import autotune, lbfgs, workloads
 from .propagator import Grid, Propagator, ShotBatch, lame, lame_adjoint
 
 
@@ -211,6 +211,12 @@ def main(argv=None):
     survey = Survey(w0, dev)
     batch = args.batch or survey.max_batch()
     f = lambda a: _tensor(a, dev)
+    # tiles for this grid and sub-batch size (cached per shape)
+    lam0, mu0 = lame(f(w0.vp), f(w0.vs), f(w0.rho))
+    tune_prop = Propagator(survey.grid, lam0, mu0, f(w0.rho), survey.batch(np.arange(batch), w0.m6))
+    log(f"[fwi] tiles: { {k: v['tile'] for k, v in autotune.tune(tune_prop).items()} }")
+    del tune_prop, lam0, mu0
+    torch.cuda.empty_cache()
     obs = observed(survey, f(wt.vp), f(wt.vs), f(wt.rho), wt.m6, batch)
     m6_start = np.tile(np.array([0.5, 0.0, 0.5, 0.0, 0.5, 0.0]), (w0.S, 1))
     vp0, vs0, rho0 = f(w0.vp), f(w0.vs), f(w0.rho)

@@ -26,4 +26,4 @@ def test_all_tiles_parity_clean_and_tuned_gradient_unchanged(gpu_lib):
     for a, b in zip(ref[2:], got[2:]):
         assert float((a.double() - b.double()).abs().max()) <= 1e-6 * float(b.double().abs().max())
     for k in range(4):
-        autotune.set_tile(k, autotune.REF_TILE)
+        autotune.set_tile(k, autotune.DEFAULT_TILE)
