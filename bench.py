@@ -96,11 +96,12 @@ def time_kernels(prop, reps=30):
     return {"stress": B.time(0, reps), "velocity": B.time(1, reps), "adjoint_step": B.time(2, reps)}
 
 
-def batched_vs_plain(wl, S, steps, chunk=50):
+def batched_vs_plain(wl, S, steps, chunk=50, tune=False):
     """Forward grid-point updates/s on a bounded sample of a multi-shot
     workload: its first S shots (record lengths scaled to at most ``steps``,
     staying uneven), advanced together in one batch (hipGraph time loop) and
-    one shot at a time (eager launches)."""
+    one shot at a time (eager launches).  tune: each path runs with the forward
+    tiles the autotuner picks for its own batch size."""
     import numpy as np
     import torch
     from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch, lame
@@ -119,7 +120,10 @@ def batched_vs_plain(wl, S, steps, chunk=50):
                        {k: v[sel] for k, v in cp.items()}, device=dev)
         return Propagator(grid, lam, mu, rho, sh)
 
+    from elastic_fm_vp_vs_rho_amd import autotune
     batch = prop(np.arange(S))
+    if tune:
+        autotune.tune(batch, kernels=(0, 1))
     batch.forward(nsteps=chunk, graph_chunk=chunk)           # warm-up (graph capture path)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -128,6 +132,8 @@ def batched_vs_plain(wl, S, steps, chunk=50):
     tb = time.perf_counter() - t0
     del batch
     singles = [prop(np.array([s])) for s in range(S)]
+    if tune:
+        autotune.tune(singles[0], kernels=(0, 1))
     singles[0].forward(nsteps=chunk)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -175,6 +181,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--c3-full", action="store_true",
+                    help="only measure the whole C3 workload (32 shots of 256x256x128, record lengths "
+                         "1500-3000) batched vs one shot at a time, with tuned tiles, and print that")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-vs-plain forward comparison on a C3 sample")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -185,8 +194,8 @@ def main():
                          "or sqrt(nt) checkpoints with segment re-forward")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every GPU box of this build has one MI355X and the path "
-              "shards by shot with no cross-GPU work; run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU (the propagator shards by shot, with no cross-GPU "
+              "work inside a gradient); run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -196,6 +205,11 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.c3_full:
+        c3 = workloads.c3()
+        print(json.dumps({"workload": c3.name, "batched_vs_plain":
+                          batched_vs_plain(c3, c3.S, int(c3.nt.max()), tune=True)}))
+        return
     wl = workloads.c2(n=args.n, nt=args.nt, anomaly=False)
     wt = workloads.c2(n=args.n, nt=args.nt, anomaly=True)
     grid = Grid(wl.nx, wl.ny, wl.nz, wl.dx, wl.dt, (wl.W,) * 6)

@@ -14,7 +14,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def gpu_lib():
-    """The C-ABI library, loaded on a GPU box.  Fails loudly if it is missing."""
+    """The C-ABI library, loaded on a machine with a GPU.  Fails loudly if it is missing."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
