@@ -203,6 +203,9 @@ def main(argv=None):
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--batch", type=int, default=0, help="shots per sub-batch (0: as many as fit)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the one-shot-at-a-time gradient")
+    ap.add_argument("--no-tune", action="store_true", help="keep the installed tiles")
+    ap.add_argument("--gradient-only", action="store_true",
+                    help="time two batched gradients and stop (no inversion)")
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
@@ -211,15 +214,26 @@ def main(argv=None):
     survey = Survey(w0, dev)
     batch = args.batch or survey.max_batch()
     f = lambda a: _tensor(a, dev)
-    # tiles for this grid and sub-batch size (cached per shape)
-    lam0, mu0 = lame(f(w0.vp), f(w0.vs), f(w0.rho))
-    tune_prop = Propagator(survey.grid, lam0, mu0, f(w0.rho), survey.batch(np.arange(batch), w0.m6))
-    log(f"[fwi] tiles: { {k: v['tile'] for k, v in autotune.tune(tune_prop).items()} }")
-    del tune_prop, lam0, mu0
-    torch.cuda.empty_cache()
+    if not args.no_tune:
+        # tiles for this grid and sub-batch size (cached per shape)
+        lam0, mu0 = lame(f(w0.vp), f(w0.vs), f(w0.rho))
+        tune_prop = Propagator(survey.grid, lam0, mu0, f(w0.rho), survey.batch(np.arange(batch), w0.m6))
+        log(f"[fwi] tiles: { {k: v['tile'] for k, v in autotune.tune(tune_prop).items()} }")
+        del tune_prop, lam0, mu0
+        torch.cuda.empty_cache()
     obs = observed(survey, f(wt.vp), f(wt.vs), f(wt.rho), wt.m6, batch)
     m6_start = np.tile(np.array([0.5, 0.0, 0.5, 0.0, 0.5, 0.0]), (w0.S, 1))
     vp0, vs0, rho0 = f(w0.vp), f(w0.vs), f(w0.rho)
+    if args.gradient_only:
+        out = {"workload": w0.name, "shots_per_batch": batch, "s_per_gradient": []}
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gradient(survey, vp0, vs0, rho0, m6_start, obs, batch)
+            torch.cuda.synchronize()
+            out["s_per_gradient"].append(time.perf_counter() - t0)
+        print(json.dumps(out))
+        return
     # s per FWI gradient, batched (sub-batches of `batch`) vs plain (one shot at a time)
     times = {}
     for name, b in (("batched", batch),) + ((("plain", 1),) if not args.no_plain else ()):
