@@ -286,6 +286,7 @@ class Elastic3D:
         self.forwFields = None
         self._borders = None
         self._shots = None
+        self._state_pool = []     # forward states of exact gradients, reused across calls
         self.mk.hostToDevice(self.device)
 
     # -- configuration ---------------------------------------------------------
@@ -324,8 +325,13 @@ class Elastic3D:
         self._shots = ShotBatch(self.grid, [(self.Sx, self.Sy, self.Sz)], self.M.as_array()[None], w,
                                 [self.nt], rec[None], self.cpml.profiles(), device=self.device)
 
+    def release_states(self):
+        """Free the forward states kept for exact gradients (no reference counterpart)."""
+        self._state_pool.clear()
+
     def _propagator(self):
-        return Propagator(self.grid, self.mk.d_Lambda, self.mk.d_Mu, self.mk.d_Rho, self._shots)
+        return Propagator(self.grid, self.mk.d_Lambda, self.mk.d_Mu, self.mk.d_Rho, self._shots,
+                          pool=self._state_pool)
 
     # -- forward (M:1812-1848) ---------------------------------------------------
     def forwardFields(self, strname="mod", saveFieldFlag=False):

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import autotune, lbfgs, workloads
-from .propagator import Grid, Propagator, ShotBatch, lame, lame_adjoint
+from .propagator import Grid, Propagator, ShotBatch, available_bytes, lame, lame_adjoint
 
 
 class Survey:
@@ -55,7 +55,7 @@ class Survey:
         return z, z, z
 
     def max_batch(self, fraction=0.8):
-        free, _ = torch.cuda.mem_get_info(self.dev)
+        free = available_bytes(self.dev)
         return max(1, min(self.S, int(fraction * free // self.shot_bytes())))
 
 

@@ -138,6 +138,15 @@ def padded_components(ncomp, N, device, pad=None):
     return torch.as_strided(buf, (ncomp, N), (N + pad, 1))
 
 
+def available_bytes(device):
+    """Device memory a new allocation can use: what the runtime reports free plus
+    what torch's caching allocator holds reserved but unused (torch hands those
+    blocks out again before asking the runtime)."""
+    free, _ = torch.cuda.mem_get_info(device)
+    cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    return free + max(0, cached)
+
+
 class Wavefield:
     """Batched state: 9 volumes (S, ny, nz, nx) + three ψ slabs.  The 9 volumes
     share one allocation with a padded field stride (S·N + pad floats), so that
@@ -206,13 +215,21 @@ def lame_adjoint(vp, vs, rho, glam, gmu, grho):
 class Propagator:
     """Forward modelling and exact-adjoint gradients for a shot batch in one medium."""
 
-    def __init__(self, grid: Grid, lam, mu, rho, shots: ShotBatch):
+    def __init__(self, grid: Grid, lam, mu, rho, shots: ShotBatch, pool=None):
+        """pool: an optional list of wavefields of this grid and shot count,
+        shared with other propagators (the forward states of a gradient are
+        taken from it and it grows as needed), so that repeated gradients do not
+        allocate and clear them again."""
         self.grid, self.shots = grid, shots
         for t in (lam, mu, rho):
             if t.dtype != torch.float32 or not t.is_contiguous() or tuple(t.shape) != grid.shape:
                 raise ValueError(f"medium arrays must be contiguous float32 {grid.shape}")
         self.lam, self.mu, self.rho = lam, mu, rho
         self.lib = _capi.lib()
+        if pool is not None:
+            if any(w.grid != grid or w.S != shots.S or w.fields.device != lam.device for w in pool[:1]):
+                raise ValueError("the state pool belongs to another grid, shot count or device")
+            self._pool = pool
 
     # -- descriptors --------------------------------------------------------
     def dims(self, sign=1.0):
@@ -283,8 +300,8 @@ class Propagator:
         if len(have) >= self.shots.ntmax + 1:
             return "store"
         if budget is None:
-            free, _ = torch.cuda.mem_get_info(self.lam.device)
-            budget = 0.85 * free
+            # states already in the pool are reused, so they count as available too
+            budget = 0.85 * (available_bytes(self.lam.device) + len(have) * self.state_bytes())
         return "store" if need <= budget else "checkpoint"
 
     def _states(self, n):
@@ -298,6 +315,7 @@ class Propagator:
         return pool[:n]
 
     def release_states(self):
+        """Drop this propagator's hold on its states (a shared pool stays with its owner)."""
         self._pool = []
         self._arrays = {}
 

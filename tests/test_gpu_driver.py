@@ -151,3 +151,38 @@ def test_fwi_vp_vs_rho_runs_and_descends(gpu_lib, gradient):
     assert len(cost) == 2 and all(np.isfinite(cost))
     if gradient == "exact":
         assert cost[1] < cost[0]
+
+
+def test_exact_gradient_through_the_driver_reuses_its_states(gpu_lib):
+    """Elastic3D.gradient_exact matches the FP64 oracle's exact gradient; a second
+    call (new model, same earthquake) reuses the kept forward states and still
+    matches; release_states frees them."""
+    e3, mdl, wav, cpml, geo = _setup(mode=1, nt=90)
+    e3.setIdxEarthquake(0)
+    e3.setStressTensor(D.StressTensor(1.0, 0.0, 1.0, 0.0, 0.0, 0.0))
+    e3.forwardFields("obs")
+    e3.setStressTensor(D.StressTensor(0.5, 0.0, 0.5, 0.0, 0.5, 0.0))
+    med, cp, src, rec = _oracle(mdl, wav, cpml, geo)
+    w = wav.waveform[:, 0]
+    c = wav.dt / mdl.dx
+    obs = np.stack([e3.obsGather.Vx, e3.obsGather.Vy, e3.obsGather.Vz]).astype(np.float64)
+    for k in range(2):
+        if k == 1:                                   # a new model: the states are reused
+            mdl2 = D.Model(mdl.nz, mdl.ny, mdl.nx, mdl.dz, mdl.dy, mdl.dx)
+            mdl2.Vp, mdl2.Vs, mdl2.Rho = mdl.Vp * np.float32(1.01), mdl.Vs.copy(), mdl.Rho.copy()
+            mdl2.compute_Lambda_Mu()
+            e3.setModel(mdl2)
+            med = O.Medium(mdl2.Lambda, mdl2.Mu, mdl2.Rho)
+        J, g = e3.gradient_exact()
+        assert len(e3._state_pool) == wav.nSamples + 1
+        hist = []
+        gr, _ = O.forward(med, cp, c, src, e3.M.as_array(), w, rec, wav.nSamples, history=hist)
+        r, Jr = O.misfit(gr, obs)
+        ref = O.exact_adjoint(med, cp, c, src, w, rec, wav.nSamples, r, hist)
+        assert abs(J - Jr) <= 1e-4 * Jr
+        for name in ("lam", "mu", "rho"):
+            e = rel_l2(g[name].double().cpu().numpy(), ref[name])
+            assert e < 1e-4, (k, name, e)
+        assert rel_l2(g["m6"], ref["m6"]) < 1e-4
+    e3.release_states()
+    assert not e3._state_pool

@@ -139,6 +139,28 @@ def test_forward_wavefield_strategy_does_not_change_the_gradient(gpu_lib):
             assert torch.equal(a, b)
 
 
+def test_strategy_budget_counts_the_allocator_cache(gpu_lib):
+    """Memory that torch's caching allocator holds but does not use counts as
+    available: after a 4 GiB tensor is freed (and kept cached), the budget
+    for keeping every forward state does not shrink, so the strategy does not
+    fall back to checkpoints."""
+    from elastic_fm_vp_vs_rho_amd.propagator import available_bytes
+    dev = torch.device("cuda")
+    torch.cuda.synchronize()
+    a0 = available_bytes(dev)
+    t = torch.empty(1 << 30, dtype=torch.float32, device=dev)
+    del t
+    free, _ = torch.cuda.mem_get_info(dev)
+    a1 = available_bytes(dev)
+    assert a1 >= a0 - (64 << 20), (a0, a1)
+    assert a1 >= free
+    p = _gradient_problem()
+    prop = propagator_of(p)
+    assert prop.choose_strategy() == "store"
+    need = (p.nt + 1) * prop.state_bytes()
+    assert prop.choose_strategy(budget=need - 1) == "checkpoint"
+
+
 def test_gpu_dot_product(gpu_lib):
     """⟨F m, r⟩ = ⟨m, Fᵀ r⟩ for the moment-tensor map, in FP32 on the device."""
     p = tiny(n=(34, 26, 22), nt=160, W=5, seed=15, nrec=12)
