@@ -15,6 +15,26 @@ import torch
 
 from .propagator import Grid, Propagator, ShotBatch, lame, lame_adjoint
 
+# Idle forward-state pools, one per (grid, shot count, device).  A forward that
+# records a tape takes the pool (so no other call can overwrite its states) and
+# its backward gives it back; repeated gradients then neither allocate nor clear
+# their states again.
+_IDLE_POOLS = {}
+
+
+def _take_pool(key):
+    return _IDLE_POOLS.pop(key, [])
+
+
+def _give_pool(key, pool):
+    if pool:
+        _IDLE_POOLS[key] = pool
+
+
+def release_state_pools():
+    """Free the forward states kept between differentiable calls."""
+    _IDLE_POOLS.clear()
+
 
 class ElasticModeling(torch.autograd.Function):
     @staticmethod
@@ -25,11 +45,13 @@ class ElasticModeling(torch.autograd.Function):
         vp, vs, rho = vp.contiguous(), vs.contiguous(), rho.contiguous()
         shots.m6.copy_(m6.detach().reshape(shots.S, 6).to(torch.float32))
         lam, mu = lame(vp, vs, rho)
-        prop = Propagator(grid, lam, mu, rho, shots)
         if not any(ctx.needs_input_grad[:4]):
-            return prop.forward()[0]          # no backward will follow: keep nothing
+            return Propagator(grid, lam, mu, rho, shots).forward()[0]   # no backward follows
+        key = (grid, shots.S, vp.device)
+        pool = _take_pool(key)
+        prop = Propagator(grid, lam, mu, rho, shots, pool=pool)
         tape = prop.record(K, strategy="checkpoint" if K is not None else "auto")
-        ctx.prop, ctx.tape = prop, tape
+        ctx.prop, ctx.tape, ctx.pool = prop, tape, (key, pool)
         ctx.save_for_backward(vp, vs, rho)
         ctx.m6_dtype, ctx.m6_shape = m6.dtype, m6.shape
         # the backward may re-run forward segments from checkpoints: it needs the
@@ -49,6 +71,8 @@ class ElasticModeling(torch.autograd.Function):
             shots.m6.copy_(m6_now)
         ctx.tape = None
         ctx.prop.release_states()
+        _give_pool(*ctx.pool)
+        ctx.pool = None
         gvp, gvs, grho_total = lame_adjoint(vp, vs, rho, glam, gmu, grho)
         return (gvp, gvs, grho_total, gm6.to(ctx.m6_dtype).reshape(ctx.m6_shape), None, None, None)
 

@@ -38,11 +38,14 @@ def test_autograd_matches_oracle_velocity_gradients(gpu_lib):
     assert rel_l2(m6.grad[0].double().cpu().numpy(), ga["m6"]) < 1e-4
 
 
-def test_autograd_reused_shot_batch_and_flat_m6(gpu_lib):
-    """Two differentiable calls on one shot batch with different moment tensors
-    (checkpointed, so the backward re-runs forward segments after the second
-    call changed the batch's M) give the gradients of two separate batches; a
-    (6,) moment tensor gets a (6,) gradient."""
+@pytest.mark.parametrize("K", [8, None])
+def test_autograd_reused_shot_batch_and_flat_m6(gpu_lib, K):
+    """Two differentiable calls on one shot batch with different moment tensors,
+    both recorded before either backward, give the gradients of two separate
+    batches: with checkpoints (K = 8) the backward re-runs forward segments after
+    the second call changed the batch's M; with every state kept (K = None) the
+    second call must not reuse the first call's states.  A (6,) moment tensor
+    gets a (6,) gradient."""
     from elastic_fm_vp_vs_rho_amd.ops import elastic_modeling
     p = tiny(n=(26, 20, 18), nt=70, W=3, seed=22, nrec=5)
     grid = grid_of(p)
@@ -59,8 +62,8 @@ def test_autograd_reused_shot_batch_and_flat_m6(gpu_lib):
         m2 = torch.tensor(mb, device=dev, requires_grad=True)
         s1 = shots_of(p, grid)
         s2 = s1 if shared else shots_of(p, grid)
-        g1 = elastic_modeling(vp, vs, rho, m1, grid, s1, K=8)
-        g2 = elastic_modeling(vp, vs, rho, m2, grid, s2, K=8)
+        g1 = elastic_modeling(vp, vs, rho, m1, grid, s1, K=K)
+        g2 = elastic_modeling(vp, vs, rho, m2, grid, s2, K=K)
         loss = 0.5 * ((g1[0] - obs) ** 2).sum() + 0.5 * ((g2[0] - obs) ** 2).sum()
         loss.backward()
         assert m1.grad.shape == (6,)
