@@ -164,10 +164,20 @@ def cpu_baseline(wl, steps):
     g, _ = O.forward(med, cp, c, tuple(wl.src[0]), wl.m6[0], w, rec, steps, threads=1, history=hist)
     O.exact_adjoint(med, cp, c, tuple(wl.src[0]), w, rec, steps, g, hist)
     dt = time.perf_counter() - t0
-    return dict(value=wl.nx * wl.ny * wl.nz * steps / dt, unit="gradient cell-steps/s", cores=1,
-                kind="port", sample=f"first {steps} of {wl.ntmax} time steps of the same gradient "
-                f"(forward with stored history + exact adjoint), {wl.nx}x{wl.ny}x{wl.nz}, FP64 NumPy, "
-                f"{dt:.1f} s")
+    out = dict(value=wl.nx * wl.ny * wl.nz * steps / dt, unit="gradient cell-steps/s", cores=1,
+               kind="port", sample=f"first {steps} of {wl.ntmax} time steps of the same gradient "
+               f"(forward with stored history + exact adjoint), {wl.nx}x{wl.ny}x{wl.nz}, FP64 NumPy, "
+               f"{dt:.1f} s")
+    # the forward alone split over the host cores (y slabs on a thread pool; at most 16,
+    # the CPU share of one GPU)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    O.forward(med, cp, c, tuple(wl.src[0]), wl.m6[0], w, rec, steps, threads=threads)
+    dtf = time.perf_counter() - t0
+    out["forward_all_cores"] = dict(value=wl.nx * wl.ny * wl.nz * steps / dtf,
+                                    unit="grid-pt updates/s", cores=threads,
+                                    sample=f"first {steps} forward steps, {dtf:.1f} s")
+    return out
 
 
 def main():
