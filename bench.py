@@ -187,7 +187,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="grid size (default 128: the C2 workload)")
     ap.add_argument("--nt", type=int, default=2000)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=16,
+                    help="time steps of the FP64 CPU baseline sample (about 10 s of CPU work at 128^3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--no-tune", action="store_true")

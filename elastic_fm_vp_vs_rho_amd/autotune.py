@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes
 import json
 import os
+import sys
 
 import torch
 
@@ -174,7 +175,8 @@ def tune(prop, kernels=(0, 1, 2, 3), reps=20, tol=1e-6, use_cache=True, verbose=
         report[KERNELS[k]] = {"tile": tile, "ms": times}
         if verbose:
             print(f"[autotune] {KERNELS[k]}: best {tile} {ok[best]:.4f} ms; "
-                  + ", ".join(f"{t}: {v if isinstance(v, str) else f'{v:.4f}'}" for t, v in times.items()))
+                  + ", ".join(f"{t}: {v if isinstance(v, str) else f'{v:.4f}'}" for t, v in times.items()),
+                  file=sys.stderr)
     if use_cache:
         cache[key] = report
         try:
