@@ -294,15 +294,34 @@ class Propagator:
 
     def choose_strategy(self, budget=None):
         """'store' keeps every forward state (no re-forward; (nt+1) states of
-        HBM), 'checkpoint' keeps √nt snapshots and rebuilds each segment."""
-        need = (self.shots.ntmax + 1) * self.state_bytes()
+        HBM); 'hybrid' keeps the states of the last m steps and √-checkpoints
+        the first nt-m (only those are run twice); 'checkpoint' keeps √nt
+        snapshots and rebuilds each segment."""
+        return self.plan(budget)[0]
+
+    def plan(self, budget=None):
+        """(strategy, m): m = number of final steps whose states are kept."""
+        nt, sb = self.shots.ntmax, self.state_bytes()
         have = getattr(self, "_pool", [])
-        if len(have) >= self.shots.ntmax + 1:
-            return "store"
+        if len(have) >= nt + 1:
+            return "store", nt
         if budget is None:
             # states already in the pool are reused, so they count as available too
-            budget = 0.85 * (available_bytes(self.lam.device) + len(have) * self.state_bytes())
-        return "store" if need <= budget else "checkpoint"
+            budget = 0.85 * (available_bytes(self.lam.device) + len(have) * sb)
+        if (nt + 1) * sb <= budget:
+            return "store", nt
+        # hybrid: (m + 1) kept states + the snapshots of the first nt - m steps
+        # (the segment buffers reuse the kept states once their sweep is done),
+        # with room left for the adjoint state, ψ ping-pong and accumulators
+        fit = int(budget // sb) - 4
+        m = fit
+        while m > 0:
+            ncp = nt - m
+            K = max(1, math.isqrt(max(ncp - 1, 0)) + 1)
+            if m + 1 + -(-ncp // K) <= fit and m >= K:
+                return "hybrid", m
+            m -= max(1, m // 64)
+        return "checkpoint", 0
 
     def _states(self, n):
         """At least n zeroed wavefields, allocated once per propagator and reused
@@ -333,10 +352,14 @@ class Propagator:
         cache[key] = ([id(w) for w in states], arr, states)
         return arr
 
-    def record(self, K=None, strategy="auto"):
-        """Forward run that keeps what the adjoint needs.  Returns a tape dict."""
+    def record(self, K=None, strategy="auto", m=None):
+        """Forward run that keeps what the adjoint needs.  Returns a tape dict.
+        K: checkpoint interval; m: steps kept by a hybrid tape (default: as many
+        as fit)."""
         if strategy == "auto":
-            strategy = self.choose_strategy()
+            strategy, m_fit = self.plan()
+            if strategy == "hybrid" and m is None:
+                m = m_fit
         nt = self.shots.ntmax
         g = self.new_gathers()
         if strategy == "store":
@@ -347,6 +370,27 @@ class Propagator:
                                               self._array(states), _ptr(g), 0, nt, _stream()),
                   "efm_forward_states")
             return dict(strategy="store", gathers=g, states=states)
+        if strategy == "hybrid":
+            m = self.plan()[1] if m is None else int(m)
+            if not 0 < m < nt:
+                raise ValueError("a hybrid tape keeps between 1 and nt - 1 steps")
+            ncp = nt - m
+            K = max(1, math.isqrt(max(ncp - 1, 0)) + 1) if K is None else int(K)
+            if m < min(K, ncp):
+                raise ValueError("a hybrid tape keeps at least one checkpoint segment of states")
+            st = self.new_wavefield()
+            snaps = []
+            for n0 in range(0, ncp, K):
+                snaps.append(st.clone())
+                self.forward(st, g, it0=n0, nsteps=min(K, ncp - n0))
+            states = self._states(m + 1)
+            states[0].copy_(st)
+            del st
+            d, md, sh = self.dims(), self.medium(), self.shots.struct()
+            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(md), ctypes.byref(sh),
+                                              self._array(states[:m + 1]), _ptr(g), ncp, m, _stream()),
+                  "efm_forward_states")
+            return dict(strategy="hybrid", gathers=g, states=states[:m + 1], snaps=snaps, K=K, ncp=ncp)
         K = self.checkpoint_interval() if K is None else int(K)
         st = self.new_wavefield()
         snaps = []
@@ -394,11 +438,14 @@ class Propagator:
         if tape["strategy"] == "store":
             sweep(self._array(tape["states"][:nt + 1]), 0, nt)
         else:
+            ncp = tape.get("ncp", nt)            # steps rebuilt from snapshots
+            if tape["strategy"] == "hybrid":
+                sweep(self._array(tape["states"]), ncp, nt)
             K, snaps = tape["K"], tape["snaps"]
-            bufs = self._states(min(K, nt) + 1)
+            bufs = self._states(min(K, ncp) + 1)
             for seg in range(len(snaps) - 1, -1, -1):
                 n0 = seg * K
-                n1 = min(n0 + K, nt)
+                n1 = min(n0 + K, ncp)
                 k = n1 - n0
                 bufs[0].copy_(snaps[seg])
                 arr = self._array(bufs[:k + 1])
@@ -421,9 +468,9 @@ class Propagator:
     def gradient_from_snapshots(self, snaps, K, resid):
         return self.backprop(dict(strategy="checkpoint", snaps=snaps, K=K), resid)
 
-    def gradient(self, obs, K=None, strategy="auto"):
+    def gradient(self, obs, K=None, strategy="auto", m=None):
         """Misfit J = ½‖d - obs‖² and its exact gradient (the FWI gradient)."""
-        tape = self.record(K, strategy)
+        tape = self.record(K, strategy, m)
         resid, J = self.misfit(tape["gathers"], obs)
         self.last_strategy = tape["strategy"]
         grads = self.backprop(tape, resid)       # queued before J is read back (no host sync)

@@ -127,11 +127,16 @@ def test_exact_adjoint_given_the_oracle_residual(gpu_lib):
 
 
 def test_forward_wavefield_strategy_does_not_change_the_gradient(gpu_lib):
-    """√nt checkpoints of any interval and keeping every state give identical bits."""
+    """√nt checkpoints of any interval, hybrid tapes (the last m steps kept, the
+    rest checkpointed) and keeping every state give identical bits."""
     p = _gradient_problem()
     prop = propagator_of(p)
     obs = torch.zeros((1, 3, p.nt, len(p.rec)), device="cuda")
     outs = [prop.gradient(obs, K=K, strategy="checkpoint") for K in (1, 7, p.nt)]
+    # hybrid tapes: the last m steps kept, the first nt - m checkpointed
+    for m in (11, p.nt // 2, p.nt - 3):
+        outs.append(prop.gradient(obs, strategy="hybrid", m=m))
+        assert prop.last_strategy == "hybrid"
     outs.append(prop.gradient(obs, strategy="store"))
     assert prop.last_strategy == "store"
     for o in outs[1:]:
@@ -143,7 +148,7 @@ def test_strategy_budget_counts_the_allocator_cache(gpu_lib):
     """Memory that torch's caching allocator holds but does not use counts as
     available: after a 4 GiB tensor is freed (and kept cached), the budget
     for keeping every forward state does not shrink, so the strategy does not
-    fall back to checkpoints."""
+    fall back to a hybrid or checkpointed tape."""
     from elastic_fm_vp_vs_rho_amd.propagator import available_bytes
     dev = torch.device("cuda")
     torch.cuda.synchronize()
@@ -158,7 +163,8 @@ def test_strategy_budget_counts_the_allocator_cache(gpu_lib):
     prop = propagator_of(p)
     assert prop.choose_strategy() == "store"
     need = (p.nt + 1) * prop.state_bytes()
-    assert prop.choose_strategy(budget=need - 1) == "checkpoint"
+    assert prop.choose_strategy(budget=need - 1) == "hybrid"
+    assert prop.choose_strategy(budget=8 * prop.state_bytes()) == "checkpoint"
 
 
 def test_gpu_dot_product(gpu_lib):

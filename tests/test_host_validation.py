@@ -63,3 +63,38 @@ def test_batch_arrays_are_row_major_per_shot():
         n = t.shape[1]
         for s in range(3):
             np.testing.assert_array_equal(flat[s * n:(s + 1) * n], np.asarray(p.cp[k], np.float32))
+
+
+def test_tape_plan_fits_the_budget():
+    """The store / hybrid / checkpoint choice for a given memory budget: every
+    state when they fit; otherwise as many final steps as fit next to the
+    checkpoints of the rest (never more states than the budget holds, and at
+    least one checkpoint segment's worth so that the kept states can serve as
+    segment buffers); √nt checkpoints when even that does not fit."""
+    import math
+    import numpy as np
+    import torch
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch
+    n, nt = 24, 400
+    grid = Grid(n, n, n, 10.0, 1e-3, (3,) * 6)
+    cp = {k: np.zeros((1, n), np.float32) for k in ("ax", "bx", "ay", "by", "az", "bz")}
+    sh = ShotBatch(grid, [(12, 12, 12)], [np.ones(6)], np.zeros((1, nt), np.float32), [nt],
+                   [[(5, 5, 5)]], cp, device="cpu", fuse_receivers=False)
+    z = torch.ones((n, n, n))
+    prop = Propagator(grid, z, z, z, sh)
+    sb = prop.state_bytes()
+    assert prop.plan((nt + 1) * sb) == ("store", nt)
+    for frac in (0.9, 0.5, 0.2, 0.12):
+        budget = frac * (nt + 1) * sb
+        strategy, m = prop.plan(budget)
+        assert strategy == "hybrid", (frac, strategy)
+        ncp = nt - m
+        K = math.isqrt(ncp - 1) + 1
+        assert 0 < m < nt and m >= K
+        assert (m + 1 + -(-ncp // K)) * sb <= budget
+        # close to the best: keeping 2 % more would not fit
+        m2 = int(m * 1.02) + 2
+        ncp2 = nt - m2
+        K2 = math.isqrt(max(ncp2 - 1, 0)) + 1
+        assert (m2 + 1 + -(-ncp2 // K2) + 4) * sb > budget
+    assert prop.plan(10 * sb)[0] == "checkpoint"
