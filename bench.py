@@ -200,9 +200,12 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only run a few dispatches of each sweep kernel and a 256 MiB copy "
                          "(for hardware-counter runs)")
-    ap.add_argument("--strategy", default="auto", choices=("auto", "store", "checkpoint"),
-                    help="forward wavefields for the adjoint: every state kept in HBM, "
-                         "or sqrt(nt) checkpoints with segment re-forward")
+    ap.add_argument("--strategy", default="auto", choices=("auto", "store", "hybrid", "checkpoint"),
+                    help="forward wavefields for the adjoint: every state kept in HBM, the last "
+                         "states kept and the first steps checkpointed, or sqrt(nt) checkpoints "
+                         "with segment re-forward")
+    ap.add_argument("--keep", type=int, default=None,
+                    help="steps whose states a hybrid tape keeps (default: as many as fit)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (the propagator shards by shot, with no cross-GPU "
@@ -252,11 +255,11 @@ def main():
 
     strategy = args.strategy
     for _ in range(args.warmup):
-        prop.gradient(obs, K, strategy)
+        prop.gradient(obs, K, strategy, args.keep)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        J, g, glam, gmu, grho, gm6 = prop.gradient(obs, K, strategy)
+        J, g, glam, gmu, grho, gm6 = prop.gradient(obs, K, strategy, args.keep)
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
     s_per_grad = T / max(args.steps, 1)

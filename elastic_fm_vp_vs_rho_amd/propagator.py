@@ -371,7 +371,11 @@ class Propagator:
                   "efm_forward_states")
             return dict(strategy="store", gathers=g, states=states)
         if strategy == "hybrid":
-            m = self.plan()[1] if m is None else int(m)
+            if m is None:
+                planned, m = self.plan()
+                if planned != "hybrid":            # asked for explicitly: keep half
+                    m = nt // 2
+            m = int(m)
             if not 0 < m < nt:
                 raise ValueError("a hybrid tape keeps between 1 and nt - 1 steps")
             ncp = nt - m
