@@ -75,6 +75,15 @@ def _tensor(a, dev):
     return torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
 
 
+def sub_batches(S, batch):
+    """Shot index ranges of at most `batch` shots, as equal as possible (16 shots
+    in batches of at most 12 go as 8 + 8), so that all sub-batches share one
+    pool of kept forward states."""
+    nb = -(-S // max(1, batch))
+    cuts = [round(i * S / nb) for i in range(nb + 1)]
+    return [np.arange(cuts[i], cuts[i + 1]) for i in range(nb) if cuts[i + 1] > cuts[i]]
+
+
 def gradient(survey: Survey, vp, vs, rho, m6, obs, batch):
     """Misfit J = ½Σ‖d - obs‖² over all shots and its exact gradient with respect
     to Vp, Vs, ρ (device volumes) and every shot's MT ((S, 6) float64), computed
@@ -85,8 +94,9 @@ def gradient(survey: Survey, vp, vs, rho, m6, obs, batch):
     gr = torch.zeros_like(vp)
     gm6 = np.zeros((survey.S, 6))
     J = 0.0
-    for b0 in range(0, survey.S, batch):
-        idx = np.arange(b0, min(b0 + batch, survey.S))
+    for idx in sub_batches(survey.S, batch):
+        for k in [k for k in survey.pools if k != len(idx)]:
+            del survey.pools[k]                  # one pool at a time: never two sets of states
         prop = Propagator(survey.grid, lam, mu, rho, survey.batch(idx, m6),
                           pool=survey.pools.setdefault(len(idx), []))
         Jb, _, a, b, c, d6 = prop.gradient(obs[idx[0]:idx[-1] + 1], strategy="auto")
@@ -105,8 +115,7 @@ def gradient(survey: Survey, vp, vs, rho, m6, obs, batch):
 def misfit(survey: Survey, vp, vs, rho, m6, obs, batch):
     lam, mu = lame(vp, vs, rho)
     J = 0.0
-    for b0 in range(0, survey.S, batch):
-        idx = np.arange(b0, min(b0 + batch, survey.S))
+    for idx in sub_batches(survey.S, batch):
         prop = Propagator(survey.grid, lam, mu, rho, survey.batch(idx, m6))
         g, _ = prop.forward()
         J += float(prop.misfit(g, obs[idx[0]:idx[-1] + 1])[1])
@@ -118,8 +127,7 @@ def observed(survey: Survey, vp, vs, rho, m6, batch):
     lam, mu = lame(vp, vs, rho)
     wl = survey.wl
     out = torch.empty((survey.S, 3, wl.ntmax, wl.rec.shape[1]), device=survey.dev)
-    for b0 in range(0, survey.S, batch):
-        idx = np.arange(b0, min(b0 + batch, survey.S))
+    for idx in sub_batches(survey.S, batch):
         prop = Propagator(survey.grid, lam, mu, rho, survey.batch(idx, m6))
         out[idx[0]:idx[-1] + 1] = prop.forward()[0]
         del prop
