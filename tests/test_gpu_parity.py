@@ -126,11 +126,19 @@ def test_exact_adjoint_given_the_oracle_residual(gpu_lib):
     assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < 1e-5
 
 
-def test_forward_wavefield_strategy_does_not_change_the_gradient(gpu_lib):
+@pytest.mark.parametrize("fuse", [True, False])
+def test_forward_wavefield_strategy_does_not_change_the_gradient(gpu_lib, fuse):
     """√nt checkpoints of any interval, hybrid tapes (the last m steps kept, the
-    rest checkpointed) and keeping every state give identical bits."""
+    rest checkpointed) and keeping every state give identical bits, with the
+    receivers sampled and injected inside the sweeps or by separate kernels."""
+    from elastic_fm_vp_vs_rho_amd.propagator import Propagator, ShotBatch
+    from tests.gpu_helpers import grid_of
     p = _gradient_problem()
-    prop = propagator_of(p)
+    grid = grid_of(p)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+    prop = Propagator(grid, f(p.lam), f(p.mu), f(p.rho),
+                      ShotBatch(grid, [p.src], [p.m6], [p.wavelet], [p.nt], [p.rec], p.cp,
+                                fuse_receivers=fuse))
     obs = torch.zeros((1, 3, p.nt, len(p.rec)), device="cuda")
     outs = [prop.gradient(obs, K=K, strategy="checkpoint") for K in (1, 7, p.nt)]
     # hybrid tapes: the last m steps kept, the first nt - m checkpointed
