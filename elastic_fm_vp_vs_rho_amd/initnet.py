@@ -17,7 +17,6 @@ matrix cores.  torch only allocates the buffers and names the stream.
 """
 from __future__ import annotations
 
-import ctypes
 from dataclasses import dataclass
 
 import numpy as np
