@@ -98,3 +98,18 @@ def test_tape_plan_fits_the_budget():
         K2 = math.isqrt(max(ncp2 - 1, 0)) + 1
         assert (m2 + 1 + -(-ncp2 // K2) + 4) * sb > budget
     assert prop.plan(10 * sb)[0] == "checkpoint"
+
+
+def test_fwi_sub_batches_are_equal_and_cover_every_shot():
+    """Sub-batches of at most `batch` shots, sizes differing by at most one, in
+    order, every shot exactly once (so one pool of kept states serves all)."""
+    import numpy as np
+    from elastic_fm_vp_vs_rho_amd.fwi import sub_batches
+    for S in (1, 3, 7, 16, 33):
+        for batch in (1, 2, 5, 12, 40):
+            parts = sub_batches(S, batch)
+            sizes = [len(p) for p in parts]
+            assert max(sizes) <= batch and max(sizes) - min(sizes) <= 1, (S, batch, sizes)
+            assert len(parts) == -(-S // batch)
+            np.testing.assert_array_equal(np.concatenate(parts), np.arange(S))
+    assert [len(p) for p in sub_batches(16, 12)] == [8, 8]
