@@ -146,9 +146,13 @@ def tune(prop, kernels=(0, 1, 2, 3), reps=20, tol=1e-6, use_cache=True, verbose=
         except (OSError, ValueError):
             cache = {}
     if use_cache and key in cache:
-        for k in kernels:
-            set_tile(k, tuple(cache[key][KERNELS[k]]["tile"]))
-        return cache[key]
+        # a cache written by another build may name tiles this library lacks: re-tune then
+        compiled = set(variants())
+        entry = cache[key]
+        if all(KERNELS[k] in entry and tuple(entry[KERNELS[k]]["tile"]) in compiled for k in kernels):
+            for k in kernels:
+                set_tile(k, tuple(entry[KERNELS[k]]["tile"]))
+            return entry
     B = _Bench(prop)
     report = {}
     vs = variants()
