@@ -209,3 +209,11 @@ def test_training_on_propagator_data_learns(gpu_lib):
     assert float(err[6:].mean()) < 0.75 * float(base[6:].mean()), (err, base)  # location
     assert float(err[8]) < 0.5 * float(base[8]), (err, base)                    # depth
     assert float(err.mean()) < float(base.mean()), (err, base)                  # all outputs
+    # the joint Vp/Vs/ρ/MT A/B on two validation shots: both runs start from the
+    # same smoothed model, every accepted L-BFGS step lowers the misfit
+    jab = D.joint_ab(c5, net, data[:2], np.arange(2), iters=3, tols=(0.5,))
+    for who in ("plain", "network"):
+        J = jab["J"][who]
+        assert 2 <= len(J) <= 4 and np.isfinite(J).all()
+        assert all(b < a for a, b in zip(J, J[1:])), J
+    assert set(jab["iters_to_tol"]) == {"plain@0.5", "network@0.5"}
