@@ -3,8 +3,9 @@
 ``elastic_modeling(vp, vs, rho, m6, grid, shots)`` returns the gathers
 (S, 3, ntmax, nrec) of a shot batch.  Its backward is the exact discrete
 adjoint of the HIP propagator (adjoint.hip): the forward keeps every state when
-they fit in HBM (else √nt checkpoints, and the backward rebuilds each
-segment), the backward runs the transposed sweeps, which
+they fit in HBM, else the states of the last steps plus √-checkpoints of the
+first (or √nt checkpoints alone), and the backward rebuilds the checkpointed
+segments; the backward runs the transposed sweeps, which
 cross-correlate forward and adjoint wavefields into ∂/∂λ, ∂/∂μ, ∂/∂ρ and
 ∂/∂M; efm_model_lame_adjoint maps them to Vp, Vs, ρ.  No torch arithmetic is
 involved on either pass.

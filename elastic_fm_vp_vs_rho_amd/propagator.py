@@ -1,5 +1,7 @@
 """Device engine over libefm.so: wavefield storage, shot batches, forward runs
-and exact-adjoint gradients with √nt checkpointing.
+and exact-adjoint gradients over a forward tape that stores every state, keeps
+the last states and √-checkpoints the first steps (hybrid), or keeps √nt
+checkpoints, whichever fits the device memory.
 
 torch is used only to own device memory and to name the current HIP stream;
 every arithmetic operation on the path runs in the HIP kernels of libefm.so.
@@ -365,8 +367,8 @@ class Propagator:
         if strategy == "store":
             states = self._states(nt + 1)
             states[0].zero_()
-            d, m, sh = self.dims(), self.medium(), self.shots.struct()
-            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
+            d, md, sh = self.dims(), self.medium(), self.shots.struct()
+            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(md), ctypes.byref(sh),
                                               self._array(states), _ptr(g), 0, nt, _stream()),
                   "efm_forward_states")
             return dict(strategy="store", gathers=g, states=states)
