@@ -107,6 +107,52 @@ def test_exact_gradient_matches_oracle(gpu_lib):
     assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < GRAD_TOL
 
 
+def test_exact_gradient_free_top_two_uneven_shots_match_oracle(gpu_lib):
+    """Free surface on top; two shots in one batch with different sources (one
+    three cells below the surface), MTs and record lengths (120 and 83 steps).
+    Each shot's gathers and ∂J/∂M against its own oracle run, the volume
+    gradients against the sum of the two oracle gradients, for the stored and
+    the √-checkpointed tape (the short shot ends inside a segment)."""
+    from elastic_fm_vp_vs_rho_amd.propagator import Propagator, ShotBatch
+    from tests.gpu_helpers import grid_of
+    p = tiny(n=(30, 20, 18), nt=120, W=4, seed=31, nrec=9, free_top=True, src=(12, 9, 3))
+    rng = np.random.default_rng(5)
+    # the second MT is scaled so that both shots weigh alike in the summed gradients
+    shots = [(p.src, p.m6, p.nt), ((19, 11, 12), 12 * rng.uniform(-1, 1, 6), 83)]
+    true = O.Medium(p.lam * 1.02, p.mu, p.rho * 0.99)
+    obs = np.zeros((2, 3, p.nt, len(p.rec)), np.float32)
+    ref = {"lam": 0.0, "mu": 0.0, "rho": 0.0}
+    g_ref, m6_ref, J_ref = [], [], 0.0
+    for s, (src, m6, nt) in enumerate(shots):
+        w = p.wavelet[:nt]
+        obs[s, :, :nt] = O.forward(true, p.cpml(), p.c, src, m6, w, p.rec, nt)[0]
+        hist = []
+        g, _ = O.forward(p.medium(), p.cpml(), p.c, src, m6, w, p.rec, nt, history=hist)
+        r, J = O.misfit(g, obs[s, :, :nt])
+        a = O.exact_adjoint(p.medium(), p.cpml(), p.c, src, w, p.rec, nt, r, hist)
+        for k in ref:
+            ref[k] = ref[k] + a[k]
+        g_ref.append(g); m6_ref.append(a["m6"]); J_ref += J
+    grid = grid_of(p)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+    batch = ShotBatch(grid, [s[0] for s in shots], [s[1] for s in shots], [p.wavelet] * 2,
+                      [s[2] for s in shots], [p.rec] * 2, p.cp)
+    prop = Propagator(grid, f(p.lam), f(p.mu), f(p.rho), batch)
+    out = {}
+    for strategy in ("store", "checkpoint"):
+        J, gg, glam, gmu, grho, gm6 = prop.gradient(f(obs), K=11, strategy=strategy)
+        out[strategy] = (glam, gmu, grho, gm6)
+        assert abs(J - J_ref) <= 1e-4 * J_ref, (J, J_ref)
+        for s, (_, _, nt) in enumerate(shots):
+            assert rel_l2(gg[s, :, :nt].double().cpu().numpy(), g_ref[s]) < SEIS_TOL
+            assert not gg[s, :, nt:].any()
+            assert rel_l2(gm6[s].cpu().numpy(), m6_ref[s]) < GRAD_TOL, s
+        for name, got in (("lam", glam), ("mu", gmu), ("rho", grho)):
+            e = rel_l2(got.double().cpu().numpy(), ref[name])
+            assert e < GRAD_TOL, (strategy, name, e)
+    assert all(torch.equal(a, b) for a, b in zip(out["store"], out["checkpoint"]))
+
+
 def test_exact_adjoint_given_the_oracle_residual(gpu_lib):
     """Adjoint precision alone: the device adjoint is driven by the oracle's
     residual, so forward FP32 error cannot leak in through the misfit."""
