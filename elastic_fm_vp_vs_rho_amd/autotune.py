@@ -35,6 +35,17 @@ def variants():
     return [tuple(buf[4 * i:4 * i + 4]) for i in range(n)]
 
 
+def shots_per_dispatch(grid):
+    """Most shots a batch may hold on `grid`: each sweep runs all shots of a
+    batch in one dispatch of fewer than 2^32 work-items, whichever compiled
+    tile is installed (the library refuses larger batches)."""
+    worst = 0
+    for bx, bz, ny, _ in variants():
+        nyc = -(-(grid.ny - 4) // ny)
+        worst = max(worst, -(-grid.nx // bx) * bx * -(-grid.nz // bz) * bz * nyc)
+    return max(1, ((1 << 32) - 1 - 8 * 512) // worst)
+
+
 def get_tile(k):
     v = [ctypes.c_int() for _ in range(4)]
     _capi.check(_capi.lib().efm_get_tile(k, *[ctypes.byref(x) for x in v]), "efm_get_tile")

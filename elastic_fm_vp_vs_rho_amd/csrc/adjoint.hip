@@ -618,6 +618,7 @@ static inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b
 extern "C" int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float *bbar,
                                   void *stream) {
     if (!d || !m || !mbar || !bbar) return fail(EFM_E_ARG, "efm_derived_medium: null argument");
+    if (int rc = check_dims(d)) return rc;
     const Geo g = make_geo(*d);
     hipLaunchKernelGGL(k_derived, dim3(blocks_for(g.N, 256)), dim3(256), 0, (hipStream_t)stream, g,
                        m->mu, m->rho, mbar, bbar);
@@ -631,6 +632,7 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
     if (!d || !m || !mbar || !bbar || !sh || !fwd || !adj || !adj2 || (!resid && d->nrec > 0) ||
         !acc || !gm6)
         return fail(EFM_E_ARG, "efm_adjoint: null argument");
+    if (int rc = check_dims(d)) return rc;
     if (n0 < 0 || n1 <= n0 || n1 > d->ntmax) return fail(EFM_E_ARG, "efm_adjoint: bad step range");
     const Geo g = make_geo(*d);
     hipStream_t st = (hipStream_t)stream;
@@ -692,6 +694,7 @@ extern "C" int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const 
                                     float *grho, void *stream) {
     if (!d || !m || !mbar || !bbar || !acc || !glam || !gmu || !grho)
         return fail(EFM_E_ARG, "efm_gradient_combine: null argument");
+    if (int rc = check_dims(d)) return rc;
     const Geo g = make_geo(*d);
     hipLaunchKernelGGL(k_combine, dim3(blocks_for(g.N, 256)), dim3(256), 0, (hipStream_t)stream, g,
                        m->mu, mbar, bbar, acc, glam, gmu, grho);

@@ -1,6 +1,9 @@
 // extern "C" entry points of libefm.so that are not defined next to their
 // kernels, plus error plumbing and the tile registry used by the autotuner.
+#include <algorithm>
 #include <atomic>
+#include <climits>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -50,11 +53,21 @@ int set_tile(int kernel, Tile t) {
     return fail(EFM_E_UNSUPPORTED, "tile not compiled into this library");
 }
 
-int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
-                      const StatePtrs &out, int it, int src_mode, hipStream_t st);
+// Work-items of one sweep dispatch with tile t (sweep_grid's launch shape,
+// counted in 64 bits).
+static long long sweep_items(const Geo &g, const Tile &t) {
+    const long long per = (long long)t.bx * t.bz;
+    if (t.map > 0) {
+        const long long B = sweep_tiles(g, t.bx, t.bz, t.ny);
+        return t.map * ((B + t.map - 1) / t.map) * per;
+    }
+    const long long nbx = (g.nx + t.bx - 1) / t.bx, nbz = (g.nz + t.bz - 1) / t.bz;
+    const long long nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
+    return nbx * nbz * nyc * g.S * per;
+}
 
-static int validate(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
-    if (!d || !m || !sh) return fail(EFM_E_ARG, "null argument");
+int check_dims(const EfmDims *d) {
+    if (!d) return fail(EFM_E_ARG, "null dimensions");
     if (d->nx < 5 || d->ny < 5 || d->nz < 5 || d->nshot < 1)
         return fail(EFM_E_ARG, "grid must be at least 5 cells per axis and nshot >= 1");
     if (d->wx_lo < 0 || d->wx_hi < 0 || d->wy_lo < 0 || d->wy_hi < 0 || d->wz_lo < 0 ||
@@ -62,6 +75,30 @@ static int validate(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
         d->wz_lo + d->wz_hi > d->nz)
         return fail(EFM_E_ARG, "CPML widths out of range");
     if (d->ntmax < 1 || d->nrec < 0) return fail(EFM_E_ARG, "ntmax must be >= 1 and nrec >= 0");
+    // the sweeps address a shot's cells and CPML slab components with 32-bit offsets
+    const long long N = (long long)d->nx * d->ny * d->nz;
+    const long long slab = 6LL * std::max({(long long)d->ny * d->nz * (d->wx_lo + d->wx_hi),
+                                           (long long)(d->wy_lo + d->wy_hi) * d->nz * d->nx,
+                                           (long long)d->ny * (d->wz_lo + d->wz_hi) * d->nx});
+    if (N > INT_MAX || slab > INT_MAX)
+        return fail(EFM_E_ARG, "grid too large: a shot's cells and its CPML slab values must each "
+                               "number below 2^31");
+    // every shot of the batch is in one dispatch per sweep
+    const Geo g = make_geo(*d);
+    for (int k = 0; k < 4; ++k)
+        if (sweep_items(g, get_tile(k)) > (long long)UINT32_MAX)
+            return fail(EFM_E_ARG, "shot batch too large for one sweep dispatch (2^32 work-items): "
+                                   "split it into smaller batches");
+    return EFM_OK;
+}
+
+int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
+                      const StatePtrs &out, int it, int src_mode, hipStream_t st);
+
+static int validate(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
+    if (!d || !m || !sh) return fail(EFM_E_ARG, "null argument");
+    int rc = check_dims(d);
+    if (rc) return rc;
     if (!m->lam || !m->mu || !m->rho) return fail(EFM_E_ARG, "medium pointer is null");
     if (!sh->ax || !sh->bx || !sh->ay || !sh->by || !sh->az || !sh->bz || !sh->src || !sh->m6 ||
         !sh->wavelet || !sh->nt || (d->nrec > 0 && !sh->rec))
@@ -132,8 +169,9 @@ extern "C" int efm_velocity_step(const EfmDims *d, const EfmMedium *m, const Efm
 extern "C" int efm_record(const EfmDims *d, const EfmShots *sh, const EfmState *st, float *gathers,
                           int it, void *stream) {
     if (!d || !sh || !gathers) return fail(EFM_E_ARG, "efm_record: null argument");
-    int rc = validate_state(st);
+    int rc = check_dims(d);
     if (rc) return rc;
+    if ((rc = validate_state(st))) return rc;
     if (it < 0 || it >= d->ntmax) return fail(EFM_E_ARG, "efm_record: it out of range");
     return launch_record(make_geo(*d), make_shots(*sh), make_state(*st), gathers, it,
                          (hipStream_t)stream);

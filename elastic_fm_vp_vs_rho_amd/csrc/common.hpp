@@ -126,6 +126,9 @@ __device__ __forceinline__ bool interior(int x, int y, int z, const Geo &g) {
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int check_launch(const char *what);
+// Grid, CPML widths, record length and the size limits of the sweep kernels
+// (32-bit offsets inside a shot, one dispatch of < 2^32 work-items per sweep).
+int check_dims(const EfmDims *d);
 
 // ---- tile configuration chosen by the autotuner ------------------------------
 struct Tile {

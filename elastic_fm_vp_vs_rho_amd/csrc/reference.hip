@@ -136,15 +136,14 @@ static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) 
 
 static int check_common(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
     if (!d || !m || !sh) return fail(EFM_E_ARG, "null argument");
-    if (d->nx < 5 || d->ny < 5 || d->nz < 5 || d->nshot < 1)
-        return fail(EFM_E_ARG, "grid must be at least 5 cells per axis and nshot >= 1");
-    return EFM_OK;
+    return check_dims(d);
 }
 
 extern "C" int efm_inject_residual(const EfmDims *d, const EfmShots *sh, const EfmState *st,
                                    const float *res, int it, void *stream) {
     if (!d || !sh || !st || (!res && d->nrec > 0))
         return fail(EFM_E_ARG, "efm_inject_residual: null argument");
+    if (int rc = check_dims(d)) return rc;
     const Geo g = make_geo(*d);
     if (g.nrec <= 0) return EFM_OK;
     hipLaunchKernelGGL(k_inject_ref, dim3(g.S), dim3(64), 0, (hipStream_t)stream, g,
@@ -191,7 +190,7 @@ extern "C" int efm_backward_mt_ref(const EfmDims *d, const EfmMedium *m, const E
 }
 
 extern "C" long long efm_border_floats(const EfmDims *d, int W) {
-    if (!d) return -1;
+    if (!d || check_dims(d)) return -1;
     return border_floats(make_geo(*d), W);
 }
 

@@ -58,7 +58,8 @@ class Survey:
 
     def max_batch(self, fraction=0.8):
         free = available_bytes(self.dev)
-        return max(1, min(self.S, int(fraction * free // self.shot_bytes())))
+        return max(1, min(self.S, autotune.shots_per_dispatch(self.grid),
+                           int(fraction * free // self.shot_bytes())))
 
     def default_batch(self):
         """One shot per sub-batch when a single shot already fills the GPU (2^21

@@ -23,6 +23,13 @@
  *                    -1 = unused slot
  *   sources          (S, 3) int32 (x, y, z); must be interior (2 <= . < n-2)
  *   moment tensors   (S, 6) float32 (xx, yy, zz, xy, xz, yz) (M:1374-1389)
+ *
+ * Size limits (checked by every entry point that takes EfmDims, EFM_E_ARG):
+ * one shot's grid below 2^31 cells and each CPML slab array below 2^31 values
+ * per shot (the kernels use 32-bit offsets within a shot); each sweep runs all
+ * shots of the batch in one dispatch of fewer than 2^32 work-items (about
+ * 4e9 cells over all shots with the default tile), so larger batches are
+ * refused and must be split.
  */
 #ifndef EFM_H
 #define EFM_H

@@ -74,6 +74,28 @@ def test_argument_validation_without_gpu(L):
                          None, None, 0, 5, 6, None) == -1   # step range past ntmax
 
 
+def test_size_limits_are_refused_without_gpu(L):
+    """Grids whose per-shot offsets would pass 2^31, and batches whose sweep
+    would pass 2^32 work-items in one dispatch, are refused by every entry point
+    that takes the dimensions, before any launch."""
+    from elastic_fm_vp_vs_rho_amd import _capi
+    from elastic_fm_vp_vs_rho_amd._capi import EfmDims, EfmMedium, EfmShots, EfmState
+    m = EfmMedium(1, 1, 1)
+    sh = EfmShots(*([1] * 11))
+    st = EfmState(*([1] * 12))
+    big = EfmDims(2048, 1100, 1000, 1, 10, 10, 10, 10, 0, 10, 100, 0, 0.1)    # 2.25e9 cells
+    many = EfmDims(512, 512, 512, 40, 10, 10, 10, 10, 0, 10, 100, 0, 0.1)     # 5.4e9 cells in all
+    for d, word in ((big, "grid too large"), (many, "shot batch too large")):
+        assert L.efm_forward(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(st),
+                             None, None, 0, 0, 1, None) == -1
+        assert word in _capi.last_error()
+        assert L.efm_derived_medium(ctypes.byref(d), ctypes.byref(m), 1, 1, None) == -1
+        assert word in _capi.last_error()
+        assert L.efm_gradient_combine(ctypes.byref(d), ctypes.byref(m), 1, 1, 1, 1, 1, 1, None) == -1
+        assert word in _capi.last_error()
+        assert L.efm_border_floats(ctypes.byref(d), 10) == -1
+
+
 def test_shot_batch_validation():
     """Host-side validation of sources/receivers happens before any upload."""
     import numpy as np
@@ -84,3 +106,19 @@ def test_shot_batch_validation():
         ShotBatch(g, [(1, 8, 8)], [np.zeros(6)], [np.zeros(10)], [10], [[(5, 5, 2)]], cp, device="cpu")
     with pytest.raises(ValueError, match="receiver"):
         ShotBatch(g, [(8, 8, 8)], [np.zeros(6)], [np.zeros(10)], [10], [[(5, 16, 2)]], cp, device="cpu")
+
+
+def test_shots_per_dispatch_fits_every_tile(L):
+    """The batch cap that sub-batching uses keeps every compiled tile's sweep
+    under 2^32 work-items, and leaves the workloads' batches whole."""
+    from elastic_fm_vp_vs_rho_amd import autotune
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid
+    for n, S_needed in (((40, 40, 40), 768), ((96, 96, 96), 256), ((256, 256, 128), 32)):
+        g = Grid(n[0], n[1], n[2], 10.0, 1e-3, (10,) * 6)
+        S = autotune.shots_per_dispatch(g)
+        assert S >= S_needed, (n, S)
+        for bx, bz, ny, mp in autotune.variants():
+            blocks = -(-g.nx // bx) * -(-g.nz // bz) * -(-(g.ny - 4) // ny) * S
+            if mp:
+                blocks = mp * -(-blocks // mp)
+            assert blocks * bx * bz < 1 << 32
