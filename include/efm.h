@@ -4,9 +4,11 @@
  * The library replaces the device side of mainPropElastic3D_MT.py (the PyCUDA
  * SourceModule kernels, M:42-530, and the per-step launch sequences of
  * Elastic3D, M:1491-2082).  Every entry point is asynchronous on the HIP
- * stream it is given, allocates nothing, and returns 0 or a negative EFM_E_*
- * code; efm_last_error() returns the message of the last failure on the
- * calling thread.
+ * stream it is given (efm_forward_graph returns after its work is done),
+ * allocates no device buffers (efm_forward_graph builds, replays and releases
+ * one HIP graph per call), and returns 0 or a negative EFM_E_* code;
+ * efm_last_error() returns the message of the last failure on the calling
+ * thread.
  *
  * Memory conventions (all pointers are device pointers, float32 unless noted):
  *   volume           (ny, nz, nx), x fastest — the reference layout (M:72, 758)
