@@ -8,5 +8,6 @@ inversion), ``autotune`` (tile autotuner), ``initnet`` / ``initnet_data``
 (learned MT/location initializer), ``model`` (CPML profiles, wavelets, Lamé
 maps), ``workloads`` (the C1–C5 configurations), ``_capi`` (ctypes binding of
 libefm.so) and ``build`` (compiles the HIP sources in place).
-Importing a module that launches kernels raises if libefm.so is not built.
+Every call that would launch a kernel raises EfmError when libefm.so is not
+built; there is no CPU fallback.
 """
