@@ -96,6 +96,44 @@ def time_kernels(prop, reps=30):
     return {"stress": B.time(0, reps), "velocity": B.time(1, reps), "adjoint_step": B.time(2, reps)}
 
 
+def sample_propagator(wl, S, steps):
+    """Propagator for the first S shots of a workload, every record cut to
+    `steps` steps, in the workload's smooth model (a kernel-profiling sample)."""
+    import numpy as np
+    import torch
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch, lame
+    dev = torch.device("cuda:0")
+    idx = np.arange(S)
+    grid = Grid(wl.nx, wl.ny, wl.nz, wl.dx, wl.dt, (wl.W,) * 6)
+    cp = {k: v[idx] for k, v in wl.cpml().items()}
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+    rho = f(wl.rho)
+    lam, mu = lame(f(wl.vp), f(wl.vs), rho)
+    sh = ShotBatch(grid, wl.src[idx], wl.m6[idx], wl.wavelets()[idx][:, :steps], np.full(S, steps),
+                   wl.rec[idx], cp, device=dev)
+    return Propagator(grid, lam, mu, rho, sh)
+
+
+def profile_kernels(prop, name):
+    """A few dispatches of each sweep kernel (after tuning, which dispatches
+    every variant) and three 256 MiB copies, for hardware-counter runs; prints
+    the kernel times and the interior cells per launch."""
+    import torch
+    from elastic_fm_vp_vs_rho_amd import autotune
+    autotune.tune(prop, use_cache=False, reps=3)
+    kt = time_kernels(prop, reps=4)
+    src = torch.ones(64 << 20, dtype=torch.float32, device=prop.lam.device)   # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    g = prop.grid
+    cells = (g.nx - 4) * (g.ny - 4) * (g.nz - 4) * prop.shots.S
+    print(json.dumps({"workload": name, "profile_kernels_ms": kt, "interior_cells_per_launch": cells,
+                      "tiles": {k: autotune.get_tile(i) for i, k in enumerate(autotune.KERNELS)},
+                      "copy_bytes_each_way": src.numel() * 4}))
+
+
 def batched_vs_plain(wl, S, steps, chunk=50, tune=False):
     """Forward grid-point updates/s on a bounded sample of a multi-shot
     workload: its first S shots (record lengths scaled to at most ``steps``,
@@ -200,6 +238,9 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only run a few dispatches of each sweep kernel and a 256 MiB copy "
                          "(for hardware-counter runs)")
+    ap.add_argument("--profile-workload", default="c2", choices=("c2", "c3x4", "c4x1"),
+                    help="with --profile-kernels: the C2 grid, 4 shots of the 256x256x128 C3 "
+                         "grid, or one shot of the 200x200x100 C4 grid")
     ap.add_argument("--strategy", default="auto", choices=("auto", "store", "hybrid", "checkpoint"),
                     help="forward wavefields for the adjoint: every state kept in HBM, the last "
                          "states kept and the first steps checkpointed, or sqrt(nt) checkpoints "
@@ -224,6 +265,10 @@ def main():
         print(json.dumps({"workload": c3.name, "batched_vs_plain":
                           batched_vs_plain(c3, c3.S, int(c3.nt.max()), tune=True)}))
         return
+    if args.profile_kernels and args.profile_workload != "c2":
+        big = workloads.c3(S=4) if args.profile_workload == "c3x4" else workloads.c4(S=1)
+        profile_kernels(sample_propagator(big, big.S, 120), big.name + f", {big.S} shot(s), 120 steps")
+        return
     wl = workloads.c2(n=args.n, nt=args.nt, anomaly=False)
     wt = workloads.c2(n=args.n, nt=args.nt, anomaly=True)
     grid = Grid(wl.nx, wl.ny, wl.nz, wl.dx, wl.dt, (wl.W,) * 6)
@@ -236,15 +281,7 @@ def main():
     prop = Propagator(grid, lam, mu, f(wl.rho), shots)
     K = prop.checkpoint_interval()
     if args.profile_kernels:
-        from elastic_fm_vp_vs_rho_amd import autotune
-        autotune.tune(prop, use_cache=False, reps=3)
-        kt = time_kernels(prop, reps=4)
-        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)   # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"profile_kernels_ms": kt, "copy_bytes_each_way": src.numel() * 4}))
+        profile_kernels(prop, wl.name)
         return
     tiles = None
     if not args.no_tune:

@@ -40,9 +40,13 @@ def shots_per_dispatch(grid):
     batch in one dispatch of fewer than 2^32 work-items, whichever compiled
     tile is installed (the library refuses larger batches)."""
     worst = 0
-    for bx, bz, ny, _ in variants():
+    for bx, bz, ny, mp in variants():
         nyc = -(-(grid.ny - 4) // ny)
-        worst = max(worst, -(-grid.nx // bx) * bx * -(-grid.nz // bz) * bz * nyc)
+        if mp == 16:    # compact interior plane
+            per = -(-(grid.nx - 4) * (grid.nz - 4) // (bx * bz)) * bx * bz
+        else:
+            per = -(-grid.nx // bx) * bx * -(-grid.nz // bz) * bz
+        worst = max(worst, per * nyc)
     return max(1, ((1 << 32) - 1 - 8 * 512) // worst)
 
 

@@ -133,7 +133,8 @@ int check_dims(const EfmDims *d);
 // ---- tile configuration chosen by the autotuner ------------------------------
 struct Tile {
     int bx, bz, ny;  // threads in x, threads in z, cells per thread along y
-    int map;         // block mapping: 0 = linear 3-D grid; k > 0 = XCD-aware over k XCDs
+    int map;         // block mapping: 0 = linear 3-D grid; 8 = XCD-aware over 8 XCDs;
+                     // 16 = XCD-aware over the compact interior plane (see tile_coords)
 };
 
 // Compiled tile variants (bx, bz, ny, map).  Whole 64-lane waves along x.
@@ -151,19 +152,32 @@ struct Tile {
     X(128, 1, 1, 8)      \
     X(128, 2, 1, 8)      \
     X(256, 1, 1, 8)      \
-    X(64, 4, 2, 8)
+    X(64, 4, 2, 8)       \
+    X(64, 1, 1, 16)      \
+    X(128, 1, 1, 16)     \
+    X(256, 1, 1, 16)     \
+    X(128, 1, 2, 16)
 
-// Number of tiles (x-blocks × z-blocks × y-chunks × shots) of a sweep.
-inline long long sweep_tiles(const Geo &g, int bx, int bz, int ny) {
-    const long long nbx = (g.nx + bx - 1) / bx, nbz = (g.nz + bz - 1) / bz;
+// XCDs the blocks of an XCD-aware mapping are dealt over.
+__host__ __device__ constexpr int map_xcds(int map) { return map == 16 ? 8 : map; }
+
+// Number of tiles (blocks × y-chunks × shots) of a sweep: x-blocks × z-blocks,
+// or, for the compact mapping, blocks of bx·bz consecutive interior (x, z) cells.
+inline long long sweep_tiles(const Geo &g, int bx, int bz, int ny, int map) {
     const long long nyc = (g.ny - 2 * H + ny - 1) / ny;
+    if (map == 16) {
+        const long long q = (long long)(g.nx - 2 * H) * (g.nz - 2 * H);
+        return (q + bx * bz - 1) / (bx * bz) * nyc * g.S;
+    }
+    const long long nbx = (g.nx + bx - 1) / bx, nbz = (g.nz + bz - 1) / bz;
     return nbx * nbz * nyc * g.S;
 }
 
 inline dim3 sweep_grid(const Geo &g, const Tile &t) {
     if (t.map > 0) {
-        const long long B = sweep_tiles(g, t.bx, t.bz, t.ny);
-        return dim3((unsigned)(t.map * ((B + t.map - 1) / t.map)));
+        const long long B = sweep_tiles(g, t.bx, t.bz, t.ny, t.map);
+        const int k = map_xcds(t.map);
+        return dim3((unsigned)(k * ((B + k - 1) / k)));
     }
     const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
     return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
@@ -174,10 +188,39 @@ inline dim3 sweep_grid(const Geo &g, const Tile &t) {
 // its own L2, so block w is given tile k_w·per + w/k: every XCD walks a
 // contiguous run of tiles (x fastest, then z, then y), and the ±z rows and ±y
 // planes a tile reads were fetched into the same L2 by its neighbours.
+// MAP = 16: the same XCD order over blocks of BX·BZ consecutive cells of the
+// interior (x, z) plane numbered x fastest with rows wrapped, so that every
+// 64-lane wave is full whatever nx is (with nx - 4 not a multiple of 64 the
+// other mappings leave part-empty waves, which cost the HBM-bound sweeps their
+// share of bytes in flight); a wave then reads at most two row segments.
+// Lanes past the last interior cell get x = -1 (the kernels return on it).
 // Returns false for the padding blocks past the last tile.
 template <int BX, int BZ, int NYC, int MAP>
 __device__ __forceinline__ bool tile_coords(const Geo &g, int &x, int &z, int &s, int &yb) {
     const int nyc = (g.ny - 2 * H + NYC - 1) / NYC;
+    if (MAP == 16) {
+        const int nxi = g.nx - 2 * H;
+        const int Q = nxi * (g.nz - 2 * H);
+        const int nqb = (Q + BX * BZ - 1) / (BX * BZ);
+        const int B = nqb * nyc * g.S;
+        const int per = (B + 7) / 8;
+        const int w = blockIdx.x;
+        const int gi = (w % 8) * per + w / 8;
+        if (gi >= B) return false;
+        const int qb = gi % nqb;
+        const int r = gi / nqb;
+        yb = r % nyc;
+        s = r / nyc;
+        const int q = qb * (BX * BZ) + threadIdx.y * BX + threadIdx.x;
+        if (q >= Q) {
+            x = z = -1;
+            return true;
+        }
+        const int zi = q / nxi;
+        z = H + zi;
+        x = H + (q - zi * nxi);
+        return true;
+    }
     if (MAP == 0) {
         x = blockIdx.x * BX + threadIdx.x;
         z = blockIdx.y * BZ + threadIdx.y;

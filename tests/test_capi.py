@@ -53,7 +53,7 @@ def test_version_and_errors(L):
     buf = (ctypes.c_int * (4 * n))()
     assert L.efm_tile_variants(buf, n) == n
     assert all(buf[4 * i] % 64 == 0 for i in range(n))   # whole 64-lane waves along x
-    assert {buf[4 * i + 3] for i in range(n)} == {0, 8}   # linear and XCD-aware mappings
+    assert {buf[4 * i + 3] for i in range(n)} == {0, 8, 16}   # linear, XCD-aware, compact
     assert L.efm_component_pad() == 1088
     # initializer kernels validate before launching
     assert L.efm_net_gemm(0, 1, 1, None, 0, 0, None, 0, 0, None, 1, None, None, 0, None) == -1
@@ -118,7 +118,10 @@ def test_shots_per_dispatch_fits_every_tile(L):
         S = autotune.shots_per_dispatch(g)
         assert S >= S_needed, (n, S)
         for bx, bz, ny, mp in autotune.variants():
-            blocks = -(-g.nx // bx) * -(-g.nz // bz) * -(-(g.ny - 4) // ny) * S
+            if mp == 16:    # compact interior plane
+                blocks = -(-(g.nx - 4) * (g.nz - 4) // (bx * bz)) * -(-(g.ny - 4) // ny) * S
+            else:
+                blocks = -(-g.nx // bx) * -(-g.nz // bz) * -(-(g.ny - 4) // ny) * S
             if mp:
-                blocks = mp * -(-blocks // mp)
+                blocks = 8 * -(-blocks // 8)
             assert blocks * bx * bz < 1 << 32
