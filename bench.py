@@ -40,28 +40,36 @@ ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 12
 # the 128^3 grid; any other size reports traffic null.
 MEASURED_TRAFFIC = {
     "stress": {
-        "64,4,1,0": 177754539, "64,1,1,0": 208426641, "64,2,1,0": 187283109, "128,1,1,0": 191200557,
-        "128,2,1,0": 171719701, "64,4,2,0": 182638955, "64,1,1,8": 155795491, "64,2,1,8": 155797538,
-        "64,4,1,8": 155788675, "64,8,1,8": 155796418, "128,1,1,8": 155797474, "128,2,1,8": 155796002,
-        "256,1,1,8": 155796226, "64,4,2,8": 155952369,
+        "128,1,1,0": 191198985, "128,2,1,0": 171723159, "64,1,1,0": 208423385,
+        "64,2,1,0": 187268713, "64,4,1,0": 177751041, "64,4,2,0": 182557248, "128,1,1,8": 155798201,
+        "128,2,1,8": 155799161, "256,1,1,8": 155796825, "64,1,1,8": 155797497,
+        "64,2,1,8": 155795513, "64,4,1,8": 155796825, "64,4,2,8": 155947562, "64,8,1,8": 155783483,
+        "128,1,1,16": 155899096, "128,1,2,16": 157325182, "256,1,1,16": 155876119,
+        "64,1,1,16": 155907031,
     },
     "velocity": {
-        "64,4,1,0": 146728292, "64,1,1,0": 174895585, "64,2,1,0": 154757509, "128,1,1,0": 159644256,
-        "128,2,1,0": 140191749, "64,4,2,0": 148158085, "64,1,1,8": 124295153, "64,2,1,8": 124306735,
-        "64,4,1,8": 124307695, "64,8,1,8": 124556884, "128,1,1,8": 124296848, "128,2,1,8": 124304752,
-        "256,1,1,8": 124298096, "64,4,2,8": 124420835,
+        "128,1,1,0": 159646615, "128,2,1,0": 140185475, "64,1,1,0": 174894202,
+        "64,2,1,0": 154751070, "64,4,1,0": 145707363, "64,4,2,0": 148077350, "128,1,1,8": 124564422,
+        "128,2,1,8": 124303521, "256,1,1,8": 124291746, "64,1,1,8": 124301729,
+        "64,2,1,8": 124307457, "64,4,1,8": 124295874, "64,4,2,8": 124739923, "64,8,1,8": 124313344,
+        "128,1,1,16": 124346592, "128,1,2,16": 125942401, "256,1,1,16": 124318529,
+        "64,1,1,16": 124373088,
     },
     "adj_velocity": {
-        "64,4,1,0": 327631894, "64,1,1,0": 391259077, "64,2,1,0": 347957214, "128,1,1,0": 362624023,
-        "128,2,1,0": 318753629, "64,4,2,0": 373751556, "64,1,1,8": 297847223, "64,2,1,8": 297807035,
-        "64,4,1,8": 296667290, "64,8,1,8": 296892641, "128,1,1,8": 298028418, "128,2,1,8": 296165906,
-        "256,1,1,8": 298346943, "64,4,2,8": 317227304,
+        "128,1,1,0": 362581049, "128,2,1,0": 318665142, "64,1,1,0": 391627612,
+        "64,2,1,0": 348080289, "64,4,1,0": 327606184, "64,4,2,0": 372751866, "128,1,1,8": 299239087,
+        "128,2,1,8": 296497331, "256,1,1,8": 299095741, "64,1,1,8": 298732069,
+        "64,2,1,8": 298310802, "64,4,1,8": 296863053, "64,4,2,8": 317320232, "64,8,1,8": 297214054,
+        "128,1,1,16": 323691474, "128,1,2,16": 359582188, "256,1,1,16": 321699862,
+        "64,1,1,16": 324354814,
     },
     "adj_stress": {
-        "64,4,1,0": 338586213, "64,1,1,0": 423083454, "64,2,1,0": 366761377, "128,1,1,0": 387776632,
-        "128,2,1,0": 329903669, "64,4,2,0": 408266097, "64,1,1,8": 299929410, "64,2,1,8": 299599687,
-        "64,4,1,8": 298649170, "64,8,1,8": 299208883, "128,1,1,8": 300253119, "128,2,1,8": 298024758,
-        "256,1,1,8": 299641922, "64,4,2,8": 330245842,
+        "128,1,1,0": 387914116, "128,2,1,0": 329973720, "64,1,1,0": 423476112,
+        "64,2,1,0": 367385614, "64,4,1,0": 339125524, "64,4,2,0": 407293678, "128,1,1,8": 301827382,
+        "128,2,1,8": 298230547, "256,1,1,8": 300470755, "64,1,1,8": 302282437,
+        "64,2,1,8": 301029388, "64,4,1,8": 298939242, "64,4,2,8": 329715352, "64,8,1,8": 299069819,
+        "128,1,1,16": 320552393, "128,1,2,16": 365129118, "256,1,1,16": 317113624,
+        "64,1,1,16": 321030783,
     },
 }
 
