@@ -93,9 +93,12 @@ int efm_last_error(char *buf, int len);
 /* ---- tile selection (no reference counterpart: the reference hard-codes
  * block=(32,2,4), M:2764-2766).  kernel: 0 stress, 1 velocity, 2 adjoint
  * velocity, 3 adjoint stress.  A tile is bx × bz threads, ny cells per thread
- * along y; map 0 launches a linear 3-D grid, map k > 0 deals contiguous runs of
- * tiles to each of k XCDs (workgroups are dispatched to XCDs round-robin).  efm_tile_variants writes the compiled tiles as
- * quadruples and returns their count.  The autotuner (autotune.py) drives these. */
+ * along y; map 0 launches a linear 3-D grid, map 8 deals contiguous runs of
+ * tiles to each of the 8 XCDs (workgroups are dispatched to XCDs round-robin),
+ * map 16 does the same over blocks of bx·bz consecutive interior (x, z) cells
+ * with rows wrapped (full 64-lane waves whatever nx is).  efm_tile_variants
+ * writes the compiled tiles as quadruples and returns their count.  The
+ * autotuner (autotune.py) drives these. */
 int efm_set_tile(int kernel, int bx, int bz, int ny, int map);
 int efm_get_tile(int kernel, int *bx, int *bz, int *ny, int *map);
 int efm_tile_variants(int *out, int max);
