@@ -40,36 +40,36 @@ ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 12
 # the 128^3 grid; any other size reports traffic null.
 MEASURED_TRAFFIC = {
     "stress": {
-        "128,1,1,0": 191198985, "128,2,1,0": 171723159, "64,1,1,0": 208423385,
-        "64,2,1,0": 187268713, "64,4,1,0": 177751041, "64,4,2,0": 182557248, "128,1,1,8": 155798201,
-        "128,2,1,8": 155799161, "256,1,1,8": 155796825, "64,1,1,8": 155797497,
-        "64,2,1,8": 155795513, "64,4,1,8": 155796825, "64,4,2,8": 155947562, "64,8,1,8": 155783483,
-        "128,1,1,16": 155899096, "128,1,2,16": 157325182, "256,1,1,16": 155876119,
-        "64,1,1,16": 155907031,
+        "128,1,1,0": 191197943, "128,2,1,0": 171725286, "64,1,1,0": 208426950,
+        "64,2,1,0": 187274175, "64,4,1,0": 177761141, "64,4,2,0": 182717143, "128,1,1,8": 155811077,
+        "128,2,1,8": 155800902, "256,1,1,8": 155797702, "64,1,1,8": 155801414,
+        "64,2,1,8": 155800678, "64,4,1,8": 155801030, "64,4,2,8": 155900573, "64,8,1,8": 155785799,
+        "128,1,1,16": 155906500, "128,1,2,16": 157177212, "256,1,1,16": 155881668,
+        "64,1,1,16": 155922244,
     },
     "velocity": {
-        "128,1,1,0": 159646615, "128,2,1,0": 140185475, "64,1,1,0": 174894202,
-        "64,2,1,0": 154751070, "64,4,1,0": 145707363, "64,4,2,0": 148077350, "128,1,1,8": 124564422,
-        "128,2,1,8": 124303521, "256,1,1,8": 124291746, "64,1,1,8": 124301729,
-        "64,2,1,8": 124307457, "64,4,1,8": 124295874, "64,4,2,8": 124739923, "64,8,1,8": 124313344,
-        "128,1,1,16": 124346592, "128,1,2,16": 125942401, "256,1,1,16": 124318529,
-        "64,1,1,16": 124373088,
+        "128,1,1,0": 159635284, "128,2,1,0": 140184002, "64,1,1,0": 174886531,
+        "64,2,1,0": 154752186, "64,4,1,0": 145709634, "64,4,2,0": 148359267, "128,1,1,8": 124559653,
+        "128,2,1,8": 124300094, "256,1,1,8": 124292671, "64,1,1,8": 124299646,
+        "64,2,1,8": 124299198, "64,4,1,8": 124292031, "64,4,2,8": 124388374, "64,8,1,8": 124299198,
+        "128,1,1,16": 124346493, "128,1,2,16": 125936803, "256,1,1,16": 124316286,
+        "64,1,1,16": 124630500,
     },
     "adj_velocity": {
-        "128,1,1,0": 362581049, "128,2,1,0": 318665142, "64,1,1,0": 391627612,
-        "64,2,1,0": 348080289, "64,4,1,0": 327606184, "64,4,2,0": 372751866, "128,1,1,8": 299239087,
-        "128,2,1,8": 296497331, "256,1,1,8": 299095741, "64,1,1,8": 298732069,
-        "64,2,1,8": 298310802, "64,4,1,8": 296863053, "64,4,2,8": 317320232, "64,8,1,8": 297214054,
-        "128,1,1,16": 323691474, "128,1,2,16": 359582188, "256,1,1,16": 321699862,
-        "64,1,1,16": 324354814,
+        "128,1,1,0": 361978527, "128,2,1,0": 318361448, "64,1,1,0": 391026011,
+        "64,2,1,0": 347621207, "64,4,1,0": 327157837, "64,4,2,0": 373229361, "128,1,1,8": 298551361,
+        "128,2,1,8": 296304217, "256,1,1,8": 298299512, "64,1,1,8": 298558144,
+        "64,2,1,8": 298189700, "64,4,1,8": 296961211, "64,4,2,8": 307501377, "64,8,1,8": 297139848,
+        "128,1,1,16": 324309946, "128,1,2,16": 343366564, "256,1,1,16": 321780098,
+        "64,1,1,16": 325192463,
     },
     "adj_stress": {
-        "128,1,1,0": 387914116, "128,2,1,0": 329973720, "64,1,1,0": 423476112,
-        "64,2,1,0": 367385614, "64,4,1,0": 339125524, "64,4,2,0": 407293678, "128,1,1,8": 301827382,
-        "128,2,1,8": 298230547, "256,1,1,8": 300470755, "64,1,1,8": 302282437,
-        "64,2,1,8": 301029388, "64,4,1,8": 298939242, "64,4,2,8": 329715352, "64,8,1,8": 299069819,
-        "128,1,1,16": 320552393, "128,1,2,16": 365129118, "256,1,1,16": 317113624,
-        "64,1,1,16": 321030783,
+        "128,1,1,0": 387067637, "128,2,1,0": 331348397, "64,1,1,0": 422321657,
+        "64,2,1,0": 368304597, "64,4,1,0": 341271877, "64,4,2,0": 409287811, "128,1,1,8": 303227882,
+        "128,2,1,8": 301568936, "256,1,1,8": 301902310, "64,1,1,8": 303021821,
+        "64,2,1,8": 302194991, "64,4,1,8": 303146738, "64,4,2,8": 333767018, "64,8,1,8": 302700156,
+        "128,1,1,16": 322531059, "128,1,2,16": 363893777, "256,1,1,16": 320776489,
+        "64,1,1,16": 323266959,
     },
 }
 

@@ -35,7 +35,9 @@ struct AdjArgs {
     const float *mbar, *bbar;      // (3, N) each
     StatePtrs fn, fn1;             // forward S^n, S^{n+1}
     StatePtrs a, a2;               // adjoint: a = live (v̄, τ̄, C in), a2 = C out
-    float *acc;                    // (S, 8, N)
+    float *acc;                    // per shot 8 padded components: λ, μ, μ_xy, μ_xz sums
+                                   // interleaved (one float4 per cell) in components 0-3,
+                                   // the ρ_x, ρ_y, ρ_z sums interleaved in 4-6, μ_yz in 7
     ShotPtrs sh;
     const float *resid;            // ∂J/∂gathers, used by R4 when the shots carry a receiver mask
     double *gm6;                   // ∂J/∂M, likewise
@@ -100,9 +102,8 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
     const float *__restrict__ Fx = A.fn.px + px_shot(g, s);
     const float *__restrict__ Fy = A.fn.py + py_shot(g, s);
     const float *__restrict__ Fz = A.fn.pz + pz_shot(g, s);
-    float *__restrict__ acc5 = A.acc + ((long long)s * 8 + 5) * g.ps;
-    float *__restrict__ acc6 = A.acc + ((long long)s * 8 + 6) * g.ps;
-    float *__restrict__ acc7 = A.acc + ((long long)s * 8 + 7) * g.ps;
+    // ρ sums: 3 consecutive floats per cell at component offset 4 of the shot
+    float *__restrict__ accr = A.acc + ((long long)s * 8 + 4) * g.ps;
 
     for (int y = y0; y < y1; ++y) {
         const int t = x + z * nx + y * sxz;
@@ -212,9 +213,11 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
             f_yzz += bz0 * LDF(&Fz[pi + csz]) + az0 * h_yzz;
             f_zz += bz0 * LDF(&Fz[pi + 2 * csz]) + az0 * h_zz;
         }
-        acc5[t] += vx0 * (f_xx + f_xyy + f_xzz);
-        acc6[t] += vy0 * (f_xyx + f_yy + f_yzz);
-        acc7[t] += vz0 * (f_xzx + f_yzy + f_zz);
+        float *pr = accr + 3 * (long long)t;
+        const float r0 = pr[0], r1 = pr[1], r2 = pr[2];
+        pr[0] = r0 + vx0 * (f_xx + f_xyy + f_xzz);
+        pr[1] = r1 + vy0 * (f_xyx + f_yy + f_yzz);
+        pr[2] = r2 + vz0 * (f_xzx + f_yzy + f_zz);
     }
 }
 
@@ -299,11 +302,10 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_stress(Geo g, AdjArgs A, int
     const float *__restrict__ Fx = A.fn.px + px_shot(g, s);
     const float *__restrict__ Fy = A.fn.py + py_shot(g, s);
     const float *__restrict__ Fz = A.fn.pz + pz_shot(g, s);
-    float *__restrict__ acc0 = A.acc + ((long long)s * 8 + 0) * g.ps;
-    float *__restrict__ acc1 = A.acc + ((long long)s * 8 + 1) * g.ps;
-    float *__restrict__ acc2 = A.acc + ((long long)s * 8 + 2) * g.ps;
-    float *__restrict__ acc3 = A.acc + ((long long)s * 8 + 3) * g.ps;
-    float *__restrict__ acc4 = A.acc + ((long long)s * 8 + 4) * g.ps;
+    // λ, μ-normal, μ_xy, μ_xz sums: one float4 per cell at the shot's base;
+    // μ_yz sum: a scalar array at component offset 7
+    float4 *__restrict__ acc4v = reinterpret_cast<float4 *>(A.acc + (long long)s * 8 * g.ps);
+    float *__restrict__ accyz = A.acc + ((long long)s * 8 + 7) * g.ps;
 
     for (int y = y0; y < y1; ++y) {
         const int t = x + z * nx + y * sxz;
@@ -444,11 +446,13 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_stress(Geo g, AdjArgs A, int
             fyz += bz0 * LDF(&Fz[pi + csz]) + az0 * d_yz;
             fzz += bz0 * LDF(&Fz[pi + 2 * csz]) + az0 * d_zz;
         }
-        acc0[t] += (txx + tyy + tzz) * (fxx + fyy + fzz);
-        acc1[t] += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
-        acc2[t] += txy * (fxy + fyx);
-        acc3[t] += txz * (fxz + fzx);
-        acc4[t] += tyz * (fyz + fzy);
+        float4 a = acc4v[t];
+        a.x += (txx + tyy + tzz) * (fxx + fyy + fzz);
+        a.y += 2.f * (txx * fxx + tyy * fyy + tzz * fzz);
+        a.z += txy * (fxy + fyx);
+        a.w += txz * (fxz + fzx);
+        acc4v[t] = a;
+        accyz[t] += tyz * (fyz + fzy);
     }
 }
 
@@ -521,23 +525,28 @@ __global__ void k_combine(Geo g, const float *mu, const float *mbar, const float
     float sl = 0.f, sm = 0.f, shear = 0.f, sr = 0.f;
     for (int s = 0; s < g.S; ++s) {
         const float *A = acc + (long long)s * 8 * N;
-        sl += A[i];
-        sm += A[N + i];
+        // component k of cell j: 0-3 interleaved at the base, 4 (μ_yz) at 7·N,
+        // 5-7 (ρ) interleaved at 4·N
+        auto AC = [&](int k, long long j) -> float {
+            return k < 4 ? A[4 * j + k] : k == 4 ? A[7 * N + j] : A[4 * N + 3 * j + (k - 5)];
+        };
+        sl += AC(0, i);
+        sm += AC(1, i);
         auto sh4 = [&](int comp, const float *mb, long long o1, bool v1, long long o2, bool v2) {
-            float r = A[comp * N + i] * mb[i] * mb[i];
-            if (v1) r += A[comp * N + i - o1] * mb[i - o1] * mb[i - o1];
-            if (v2) r += A[comp * N + i - o2] * mb[i - o2] * mb[i - o2];
-            if (v1 && v2) r += A[comp * N + i - o1 - o2] * mb[i - o1 - o2] * mb[i - o1 - o2];
+            float r = AC(comp, i) * mb[i] * mb[i];
+            if (v1) r += AC(comp, i - o1) * mb[i - o1] * mb[i - o1];
+            if (v2) r += AC(comp, i - o2) * mb[i - o2] * mb[i - o2];
+            if (v1 && v2) r += AC(comp, i - o1 - o2) * mb[i - o1 - o2] * mb[i - o1 - o2];
             return r;
         };
         shear += sh4(2, mxy, 1, x > 0, sxz, y > 0);
         shear += sh4(3, mxz, 1, x > 0, nx, z > 0);
         shear += sh4(4, myz, sxz, y > 0, nx, z > 0);
-        float r = A[5 * N + i] * bbx[i] * bbx[i] + A[6 * N + i] * bby[i] * bby[i]
-                + A[7 * N + i] * bbz[i] * bbz[i];
-        if (x > 0) r += A[5 * N + i - 1] * bbx[i - 1] * bbx[i - 1];
-        if (y > 0) r += A[6 * N + i - sxz] * bby[i - sxz] * bby[i - sxz];
-        if (z > 0) r += A[7 * N + i - nx] * bbz[i - nx] * bbz[i - nx];
+        float r = AC(5, i) * bbx[i] * bbx[i] + AC(6, i) * bby[i] * bby[i]
+                + AC(7, i) * bbz[i] * bbz[i];
+        if (x > 0) r += AC(5, i - 1) * bbx[i - 1] * bbx[i - 1];
+        if (y > 0) r += AC(6, i - sxz) * bby[i - sxz] * bby[i - sxz];
+        if (z > 0) r += AC(7, i - nx) * bbz[i - nx] * bbz[i - nx];
         sr += r;
     }
     const float m = mu[i];
