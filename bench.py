@@ -1,8 +1,10 @@
 #!/usr/bin/env python3
 """Benchmark: exact FWI gradient (forward + adjoint) of the 128³ three-layer
 model with a Gaussian anomaly, 2000 steps, 64 receivers, one MT source, on one
-MI355X.  A bench step is one full gradient evaluation: forward run with √nt
-checkpoints, misfit, reverse sweep (segment re-forward + adjoint + imaging),
+MI355X.  A bench step is one full gradient evaluation: forward run keeping what
+the adjoint needs (every state when they fit in HBM, as at 128³; else the last
+states plus √-checkpoints of the first steps, or √nt checkpoints), misfit,
+reverse sweep (adjoint + imaging, re-running any checkpointed segment),
 gradient assembly.
 
 Prints one JSON line.  ``value`` is gradient cell-steps per second (grid cells

@@ -125,3 +125,15 @@ def test_shots_per_dispatch_fits_every_tile(L):
             if mp:
                 blocks = 8 * -(-blocks // 8)
             assert blocks * bx * bz < 1 << 32
+
+
+def test_bench_traffic_table_covers_every_compiled_tile(L):
+    """bench.py reports the roofline's HBM traffic from measured per-tile
+    values; every tile the library compiles must have one for every sweep, or
+    the bench line would carry a null traffic whichever tile the tuner picks."""
+    import bench
+    from elastic_fm_vp_vs_rho_amd import autotune
+    keys = {",".join(str(v) for v in t) for t in autotune.variants()}
+    for k in autotune.KERNELS:
+        assert set(bench.MEASURED_TRAFFIC[k]) == keys, k
+        assert all(v > 0 for v in bench.MEASURED_TRAFFIC[k].values())
