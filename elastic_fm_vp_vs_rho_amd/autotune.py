@@ -23,7 +23,7 @@ from .propagator import Wavefield, _ptr, _stream, padded_components
 
 KERNELS = ("stress", "velocity", "adj_velocity", "adj_stress")
 REF_TILE = (64, 4, 1, 0)          # the parity reference every candidate is compared with
-DEFAULT_TILE = (128, 1, 1, 8)     # what the library installs until a tile is set
+DEFAULT_TILE = (128, 1, 1, 16)    # what the library installs until a tile is set
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "efm_tiles.json")
 
 

@@ -28,11 +28,12 @@ int check_launch(const char *what) {
 }
 
 // Tiles: 0 stress, 1 velocity, 2 adjoint velocity, 3 adjoint stress.  The
-// default, 128 x 1 threads in the XCD-aware order, measured within a few per
-// cent of the autotuner's choice for every kernel at 128^3, 256x256x128 and
-// 200x200x100.
+// default, blocks of 128 consecutive interior cells in the XCD-aware order
+// (map 16), keeps every wave full whatever nx is: measured within 0-2 % of the
+// best row tile at 128^3, 4-8 % behind it where rows fill whole waves
+// (256 wide), and 15-25 % ahead where they do not (200 and 132 wide).
 static std::mutex g_tile_mu;
-static Tile g_tiles[4] = {{128, 1, 1, 8}, {128, 1, 1, 8}, {128, 1, 1, 8}, {128, 1, 1, 8}};
+static Tile g_tiles[4] = {{128, 1, 1, 16}, {128, 1, 1, 16}, {128, 1, 1, 16}, {128, 1, 1, 16}};
 #define EFM_TILE_ENTRY(a, b, c, o) {a, b, c, o},
 static const Tile kTiles[] = {EFM_TILE_LIST(EFM_TILE_ENTRY)};
 #undef EFM_TILE_ENTRY
