@@ -64,11 +64,13 @@ def rho_nafe_drake(vp):
 
 def cpml_profile(frequency, dt, di, W, vmax, N):
     """CPML.CPML_i, M:1340-1372: float32 (a, b) profiles of length N."""
+    a = np.zeros([N], np.float32)
+    b = np.zeros([N], np.float32)
+    if W <= 0:                     # no absorbing layer: a = b = 0 everywhere
+        return a, b
     R = 1e-6
     Li = W * di
     d0 = -3 * np.log(R) / (2.0 * Li)
-    a = np.zeros([N], np.float32)
-    b = np.zeros([N], np.float32)
     for ix in range(N):
         if ix < W:
             Fi = (W - 1 - ix) * di

@@ -20,6 +20,9 @@ from oracle import elastic_fp64 as O
 from oracle import twin
 from tests.problems import rel_l2, run_forward, tiny
 
+# a division by zero or an overflow inside the oracle is a bug, not a warning
+pytestmark = pytest.mark.filterwarnings("error::RuntimeWarning")
+
 
 def _random_state(shape, rng, scale=1.0):
     F = O.Fields(shape)

@@ -1,0 +1,272 @@
+"""Pins for the oracle's reference-style imaging conditions and CPML profiles,
+from the mathematics they implement rather than from a second transcription:
+
+* kernel_Gradients_Mij (M:337-372) applies the isotropic compliance to the
+  adjoint stress at the source: for τ = C:ε (C the 6x6 Voigt stiffness built
+  from λ, μ) it must return w·(ε_xx, ε_yy, ε_zz, γ_xy, γ_xz, γ_yz), and its
+  shear moduli are the 4-cell harmonic means around each staggered shear node;
+* kernel_Gradients_Rho / _Lambda / _Mu (M:191-294) are the isotropic
+  sensitivity kernels K_ρ = -b̄·∂v/∂t, K_λ = tr(ε̄)·div v, K_μ = 2 ε̄:ε(v) with
+  ε̄ = C⁻¹:τ̄; on polynomial velocity fields their finite differences have closed
+  forms that fix the direction and width of every difference (the reference's
+  2-cell averaged backward divergence divided by dx, and its raw, undivided
+  differences in K_μ);
+* CPML.CPML_i (M:1340-1372): the damping recovered from (a, b) is the quadratic
+  profile whose theoretical normal-incidence reflection is R = 1e-6, the
+  frequency shift α falls linearly from πf at the inner edge to πf/W at the
+  outer cell, and the inner edge has a = 0, b = e^{-πf·dt}.
+
+Each check is chosen so that a dropped term, a flipped sign, a shifted index or
+a transposed operand fails it.  Both the oracle and the product's host profile
+(model.cpml_profile) are held to the CPML pins.
+"""
+import numpy as np
+import pytest
+from scipy.stats import hmean
+
+from elastic_fm_vp_vs_rho_amd import model as Mo
+from oracle import elastic_fp64 as O
+
+pytestmark = pytest.mark.filterwarnings("error::RuntimeWarning")
+
+VOIGT = ("xx", "yy", "zz", "xy", "xz", "yz")
+
+
+def _stiffness(lam, mu):
+    """6x6 isotropic Voigt stiffness (engineering shear strains γ = 2ε)."""
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[np.arange(3), np.arange(3)] = lam + 2 * mu
+    C[np.arange(3, 6), np.arange(3, 6)] = mu
+    return C
+
+
+def _strain_tensor(e6):
+    """Symmetric 3x3 strain from Voigt (ε_xx, ε_yy, ε_zz, γ_xy, γ_xz, γ_yz)."""
+    exx, eyy, ezz, gxy, gxz, gyz = e6
+    return np.array([[exx, gxy / 2, gxz / 2], [gxy / 2, eyy, gyz / 2], [gxz / 2, gyz / 2, ezz]])
+
+
+# ---------------------------------------------------------------------------
+# kernel_Gradients_Mij
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_image_mt_is_the_isotropic_compliance(seed):
+    rng = np.random.default_rng(seed)
+    shape = (7, 6, 8)
+    lam0, mu0 = 5e9 + 4e9 * rng.random(), 3e9 + 4e9 * rng.random()
+    lam = np.full(shape, lam0); mu = np.full(shape, mu0)
+    src = (3, 2, 3)   # (x, y, z)
+    eps = rng.standard_normal(6)
+    tau = _stiffness(lam0, mu0) @ eps
+    B = O.Fields(shape)
+    for k, v in zip(VOIGT, tau):
+        B.f["t" + k][src[1], src[2], src[0]] = v
+    w = -0.7
+    got = O.image_mt(B, lam, mu, src, w)
+    np.testing.assert_allclose(got, w * eps, rtol=1e-12, atol=0)
+    # the same through an independent inverse of the stiffness matrix
+    np.testing.assert_allclose(got, w * np.linalg.inv(_stiffness(lam0, mu0)) @ tau, rtol=1e-12)
+
+
+def test_image_mt_shear_moduli_are_staggered_harmonic_means():
+    """τ_xy lives at (x+½, y+½): its modulus is the harmonic mean of μ over the
+    four cells (x|x+1, y|y+1) at the source's z, and likewise for xz and yz.  The
+    diagonal terms use λ, μ of the source cell only."""
+    rng = np.random.default_rng(4)
+    shape = (7, 6, 8)
+    mu = 3e9 + 4e9 * rng.random(shape)
+    lam = 5e9 + 4e9 * rng.random(shape)
+    sx, sy, sz = 3, 2, 3
+    B = O.Fields(shape)
+    tau = rng.standard_normal(6)
+    for k, v in zip(VOIGT, tau):
+        B.f["t" + k][sy, sz, sx] = v
+    got = O.image_mt(B, lam, mu, (sx, sy, sz), 1.0)
+    m_xy = hmean([mu[sy, sz, sx], mu[sy, sz, sx + 1], mu[sy + 1, sz, sx], mu[sy + 1, sz, sx + 1]])
+    m_xz = hmean([mu[sy, sz, sx], mu[sy, sz, sx + 1], mu[sy, sz + 1, sx], mu[sy, sz + 1, sx + 1]])
+    m_yz = hmean([mu[sy, sz, sx], mu[sy + 1, sz, sx], mu[sy, sz + 1, sx], mu[sy + 1, sz + 1, sx]])
+    np.testing.assert_allclose(got[3:], tau[3:] / np.array([m_xy, m_xz, m_yz]), rtol=1e-12)
+    S = np.linalg.inv(_stiffness(lam[sy, sz, sx], mu[sy, sz, sx]))
+    np.testing.assert_allclose(got[:3], (S @ tau)[:3], rtol=1e-12)
+    # and they are the moduli the forward stress sweep uses (Medium's μ̄)
+    med = O.Medium(lam, mu, np.ones(shape))
+    np.testing.assert_allclose([med.mu_xy[sy, sz, sx], med.mu_xz[sy, sz, sx], med.mu_yz[sy, sz, sx]],
+                               [m_xy, m_xz, m_yz], rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# kernel_Gradients_Rho / _Lambda / _Mu
+# ---------------------------------------------------------------------------
+SHAPE = (9, 10, 11)            # (ny, nz, nx)
+DX, DY, DZ, DT = 7.0, 7.0, 7.0, 1e-3
+
+
+def _grids():
+    ny, nz, nx = SHAPE
+    return np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij")   # Y, Z, X
+
+
+def _interior():
+    ny, nz, nx = SHAPE
+    return (slice(2, ny - 2), slice(2, nz - 2), slice(2, nx - 2))
+
+
+def _vel(**kw):
+    """v_present dict with every component (zero unless given)."""
+    return {k: kw.get(k, np.zeros(SHAPE)) for k in ("vx", "vy", "vz")}
+
+
+def _fields(vel=None, tau=None):
+    F = O.Fields(SHAPE)
+    for k, v in (vel or {}).items():
+        F.f[k][...] = v
+    for k, v in (tau or {}).items():
+        F.f[k][...] = v
+    return F
+
+
+def _homogeneous(lam0=6e9, mu0=4e9, rho0=2300.0):
+    return O.Medium(np.full(SHAPE, lam0), np.full(SHAPE, mu0), np.full(SHAPE, rho0))
+
+
+def test_image_ref_rho_is_minus_adjoint_velocity_times_acceleration():
+    rng = np.random.default_rng(0)
+    med = _homogeneous()
+    v = {k: rng.standard_normal(SHAPE) for k in ("vx", "vy", "vz")}
+    acc = {k: rng.standard_normal(SHAPE) for k in ("vx", "vy", "vz")}
+    vbar = {k: rng.standard_normal(SHAPE) for k in ("vx", "vy", "vz")}
+    Fw = _fields(vel=v)
+    Vpres = {k: v[k] + DT * acc[k] for k in v}
+    B = _fields(vel=vbar)
+    _, _, grho = O._image_ref(B, Fw, Vpres, med, DT, DX, DY, DZ)
+    expect = -sum(vbar[k] * acc[k] for k in v)
+    I = _interior()
+    np.testing.assert_allclose(grho[I], expect[I], rtol=1e-6, atol=1e-9)
+    mask = np.ones(SHAPE, bool); mask[I] = False
+    assert not np.any(grho[mask])                       # guard cells untouched
+
+
+def test_image_ref_lambda_uses_two_cell_averaged_backward_divergence():
+    """v_x = x², v_y = y², v_z = z² (index units): the 2-cell averaged backward
+    difference is (x² - (x-2)²)/(2dx) = (2x - 2)/dx; with τ̄ = C:ε̄ the λ image
+    is tr(ε̄)·div v."""
+    Y, Z, X = _grids()
+    lam0, mu0 = 6e9, 4e9
+    med = _homogeneous(lam0, mu0)
+    eps = np.array([0.3, -0.2, 0.5, 0.1, -0.4, 0.25])
+    tau = _stiffness(lam0, mu0) @ eps
+    Vpres = _vel(vx=X ** 2.0, vy=1.5 * Y ** 2.0, vz=-0.5 * Z ** 2.0)
+    B = _fields(tau={"t" + k: np.full(SHAPE, v) for k, v in zip(VOIGT, tau)})
+    glam, _, _ = O._image_ref(B, _fields(), Vpres, med, DT, DX, DY, DZ)
+    div = (2 * X - 2) / DX + 1.5 * (2 * Y - 2) / DY - 0.5 * (2 * Z - 2) / DZ
+    I = _interior()
+    np.testing.assert_allclose(glam[I], (eps[:3].sum() * div)[I], rtol=1e-12, atol=1e-15)
+
+
+def test_image_ref_mu_is_twice_the_strain_double_dot_product():
+    """Uniform velocity gradient G (v_i = Σ_j G_ij·j in index units) and
+    uniform τ̄ = C:ε̄ in a homogeneous medium: the μ image is 2 ε̄:sym(G) with the
+    reference's undivided differences."""
+    Y, Z, X = _grids()
+    lam0, mu0 = 6e9, 4e9
+    med = _homogeneous(lam0, mu0)
+    rng = np.random.default_rng(1)
+    G = rng.standard_normal((3, 3))          # rows: v_x, v_y, v_z; columns: x, y, z
+    coords = (X, Y, Z)
+    Vpres = {k: sum(G[i, j] * coords[j] for j in range(3)) for i, k in enumerate(("vx", "vy", "vz"))}
+    eps = rng.standard_normal(6)
+    tau = _stiffness(lam0, mu0) @ eps
+    B = _fields(tau={"t" + k: np.full(SHAPE, v) for k, v in zip(VOIGT, tau)})
+    _, gmu, _ = O._image_ref(B, _fields(), Vpres, med, DT, DX, DY, DZ)
+    expect = 2.0 * np.sum(_strain_tensor(eps) * 0.5 * (G + G.T))
+    I = _interior()
+    np.testing.assert_allclose(gmu[I], expect, rtol=1e-10)
+
+
+@pytest.mark.parametrize("comp", ["vx", "vy", "vz"])
+def test_image_ref_mu_difference_directions(comp):
+    """Quadratic fields separate forward from backward differences: the normal
+    terms difference backward (x² → 2x - 1), the shear terms forward
+    (y² → 2y + 1), and the shear modulus is the staggered harmonic mean."""
+    Y, Z, X = _grids()
+    rng = np.random.default_rng(2)
+    ny, nz, nx = SHAPE
+    mu = 3e9 + 2e9 * rng.random(SHAPE)
+    lam = 5e9 + 2e9 * rng.random(SHAPE)
+    med = O.Medium(lam, mu, np.full(SHAPE, 2300.0))
+    I = _interior()
+    own = {"vx": X, "vy": Y, "vz": Z}[comp]
+    # normal term alone: only τ̄_ii of the same axis is non-zero
+    tname = {"vx": "txx", "vy": "tyy", "vz": "tzz"}[comp]
+    B = _fields(tau={tname: np.ones(SHAPE)})
+    _, gmu, _ = O._image_ref(B, _fields(), _vel(**{comp: own ** 2.0}), med, DT, DX, DY, DZ)
+    S_ii = (lam + mu) / (mu * (3 * lam + 2 * mu))       # compliance diagonal
+    np.testing.assert_allclose(gmu[I], (2 * S_ii * (2 * own - 1))[I], rtol=1e-12)
+    # shear terms: v_comp varying along each other axis, with τ̄ on that plane
+    others = {"vx": (("y", Y, "txy"), ("z", Z, "txz")),
+              "vy": (("x", X, "txy"), ("z", Z, "tyz")),
+              "vz": (("x", X, "txz"), ("y", Y, "tyz"))}[comp]
+    for axis, coord, tn in others:
+        B = _fields(tau={tn: np.ones(SHAPE)})
+        _, gmu, _ = O._image_ref(B, _fields(), _vel(**{comp: coord ** 2.0}), med, DT, DX, DY, DZ)
+        plane = tn[1:]
+        mbar = np.zeros(SHAPE)
+        off = {"x": (0, 0, 1), "y": (1, 0, 0), "z": (0, 1, 0)}
+        a, b = off[plane[0]], off[plane[1]]
+        for y in range(2, ny - 2):
+            for z in range(2, nz - 2):
+                for x in range(2, nx - 2):
+                    cells = [(y, z, x), (y + a[0], z + a[1], x + a[2]), (y + b[0], z + b[1], x + b[2]),
+                             (y + a[0] + b[0], z + a[1] + b[1], x + a[2] + b[2])]
+                    mbar[y, z, x] = hmean([mu[c] for c in cells])
+        np.testing.assert_allclose(gmu[I], (2 * coord + 1)[I] / mbar[I], rtol=1e-12, err_msg=axis)
+
+
+# ---------------------------------------------------------------------------
+# CPML.CPML_i
+# ---------------------------------------------------------------------------
+IMPLS = {"oracle": O.cpml_profile, "product": Mo.cpml_profile}
+
+
+def _recover(a, b, dt):
+    """(d, α) from b = exp(-(d+α)dt), a = d/(d+α)·(b-1)."""
+    a = a.astype(np.float64); b = b.astype(np.float64)
+    s = -np.log(b) / dt                  # d + α
+    d = a * s / (b - 1.0)
+    return d, s - d
+
+
+@pytest.mark.parametrize("impl", sorted(IMPLS))
+@pytest.mark.parametrize("W,N,f,dt,di,vmax", [(10, 64, 10.0, 1.6e-3, 10.0, 3000.0),
+                                             (20, 100, 0.1, Mo.REF_DT, 1e3, 4500.0),
+                                             (6, 30, 25.0, 4e-4, 5.0, 6000.0)])
+def test_cpml_profile_physics(impl, W, N, f, dt, di, vmax):
+    a, b = IMPLS[impl](f, dt, di, W, vmax, N)
+    assert a.dtype == np.float32 and b.dtype == np.float32
+    L = W * di
+    # inner edge of each layer: no damping, pure frequency shift α = πf
+    for i in (W - 1, N - W):
+        assert a[i] == 0.0
+        assert b[i] == np.float32(np.exp(-np.pi * f * dt))
+    assert not np.any(a[W:N - W]) and not np.any(b[W:N - W])
+    # both sides: distance into the layer F = (W-1-i)·di (low), (i-(N-W))·di (high)
+    F = np.concatenate([(W - 1 - np.arange(W)) * di, np.arange(W) * di])
+    idx = np.concatenate([np.arange(W), np.arange(N - W, N)])
+    d, alpha = _recover(a[idx], b[idx], dt)
+    # a = d(b-1)/(d+α) < 0 wherever d > 0
+    assert np.all(a[idx][F > 0] < 0)
+    # α falls linearly from πf (F = 0) to πf/W (outermost cell): α = πf(L - F)/L
+    np.testing.assert_allclose(alpha, np.pi * f * (L - F) / L, rtol=2e-3)
+    # d is quadratic in F, d = c2·F², and its continuous profile has the
+    # theoretical normal-incidence reflection R = exp(-(2/vmax)∫₀ᴸ d) = 1e-6
+    c2 = d[F > 0] / F[F > 0] ** 2
+    np.testing.assert_allclose(c2, np.median(c2), rtol=2e-3)
+    R = np.exp(-(2.0 / vmax) * np.median(c2) * L ** 3 / 3.0)
+    np.testing.assert_allclose(R, 1e-6, rtol=2e-2)
+
+
+@pytest.mark.parametrize("impl", sorted(IMPLS))
+def test_cpml_profile_without_layer(impl):
+    a, b = IMPLS[impl](10.0, 1e-3, 10.0, 0, 3000.0, 12)
+    assert not np.any(a) and not np.any(b)
