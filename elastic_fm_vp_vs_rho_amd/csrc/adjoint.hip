@@ -626,7 +626,8 @@ static inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b
 
 extern "C" int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float *bbar,
                                   void *stream) {
-    if (!d || !m || !mbar || !bbar) return fail(EFM_E_ARG, "efm_derived_medium: null argument");
+    if (!d || !m || !mbar || !bbar || !m->mu || !m->rho)
+        return fail(EFM_E_ARG, "efm_derived_medium: null argument");
     if (int rc = check_dims(d)) return rc;
     const Geo g = make_geo(*d);
     hipLaunchKernelGGL(k_derived, dim3(blocks_for(g.N, 256)), dim3(256), 0, (hipStream_t)stream, g,
@@ -641,8 +642,15 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
     if (!d || !m || !mbar || !bbar || !sh || !fwd || !adj || !adj2 || (!resid && d->nrec > 0) ||
         !acc || !gm6)
         return fail(EFM_E_ARG, "efm_adjoint: null argument");
-    if (int rc = check_dims(d)) return rc;
+    if (int rc = validate(d, m, sh)) return rc;
     if (n0 < 0 || n1 <= n0 || n1 > d->ntmax) return fail(EFM_E_ARG, "efm_adjoint: bad step range");
+    if (int rc = validate_state(d, adj)) return rc;
+    // adj2 carries only the second ψ buffer
+    if ((d->wx_lo + d->wx_hi > 0 && !adj2->psi_x) || (d->wy_lo + d->wy_hi > 0 && !adj2->psi_y) ||
+        (d->wz_lo + d->wz_hi > 0 && !adj2->psi_z))
+        return fail(EFM_E_ARG, "efm_adjoint: adj2 psi slab pointer is null");
+    for (int k = 0; k <= n1 - n0; ++k)
+        if (int rc = validate_state(d, fwd[k])) return rc;
     const Geo g = make_geo(*d);
     hipStream_t st = (hipStream_t)stream;
     AdjArgs A;
@@ -701,7 +709,7 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
 extern "C" int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const float *mbar,
                                     const float *bbar, const float *acc, float *glam, float *gmu,
                                     float *grho, void *stream) {
-    if (!d || !m || !mbar || !bbar || !acc || !glam || !gmu || !grho)
+    if (!d || !m || !m->mu || !mbar || !bbar || !acc || !glam || !gmu || !grho)
         return fail(EFM_E_ARG, "efm_gradient_combine: null argument");
     if (int rc = check_dims(d)) return rc;
     const Geo g = make_geo(*d);

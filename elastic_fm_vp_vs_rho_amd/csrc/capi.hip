@@ -97,21 +97,30 @@ int check_dims(const EfmDims *d) {
 int launch_stress_src(const Geo &g, const EfmMedium &m, const ShotPtrs &sh, const StatePtrs &in,
                       const StatePtrs &out, int it, int src_mode, hipStream_t st);
 
-static int validate(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
-    if (!d || !m || !sh) return fail(EFM_E_ARG, "null argument");
-    int rc = check_dims(d);
-    if (rc) return rc;
-    if (!m->lam || !m->mu || !m->rho) return fail(EFM_E_ARG, "medium pointer is null");
+int validate_shots(const EfmDims *d, const EfmShots *sh) {
+    if (!sh) return fail(EFM_E_ARG, "null shot description");
     if (!sh->ax || !sh->bx || !sh->ay || !sh->by || !sh->az || !sh->bz || !sh->src || !sh->m6 ||
         !sh->wavelet || !sh->nt || (d->nrec > 0 && !sh->rec))
         return fail(EFM_E_ARG, "shot pointer is null");
     return EFM_OK;
 }
 
-static int validate_state(const EfmState *s) {
+int validate(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
+    if (!d || !m || !sh) return fail(EFM_E_ARG, "null argument");
+    int rc = check_dims(d);
+    if (rc) return rc;
+    if (!m->lam || !m->mu || !m->rho) return fail(EFM_E_ARG, "medium pointer is null");
+    return validate_shots(d, sh);
+}
+
+int validate_state(const EfmDims *d, const EfmState *s) {
     if (!s || !s->vx || !s->vy || !s->vz || !s->txx || !s->tyy || !s->tzz || !s->txy ||
         !s->txz || !s->tyz)
         return fail(EFM_E_ARG, "state pointer is null");
+    // the ψ slabs of an axis are touched whenever that axis has a CPML layer
+    if ((d->wx_lo + d->wx_hi > 0 && !s->psi_x) || (d->wy_lo + d->wy_hi > 0 && !s->psi_y) ||
+        (d->wz_lo + d->wz_hi > 0 && !s->psi_z))
+        return fail(EFM_E_ARG, "state psi slab pointer is null");
     return EFM_OK;
 }
 
@@ -155,7 +164,7 @@ extern "C" int efm_tile_variants(int *out, int max) {
 extern "C" int efm_stress_step(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
                                const EfmState *in, const EfmState *out, int it, void *stream) {
     int rc = validate(d, m, sh);
-    if (rc || (rc = validate_state(in)) || (rc = validate_state(out))) return rc;
+    if (rc || (rc = validate_state(d, in)) || (rc = validate_state(d, out))) return rc;
     return launch_stress_src(make_geo(*d), *m, make_shots(*sh), make_state(*in), make_state(*out),
                              it, 1, (hipStream_t)stream);
 }
@@ -163,7 +172,7 @@ extern "C" int efm_stress_step(const EfmDims *d, const EfmMedium *m, const EfmSh
 extern "C" int efm_velocity_step(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
                                  const EfmState *in, const EfmState *out, int it, void *stream) {
     int rc = validate(d, m, sh);
-    if (rc || (rc = validate_state(in)) || (rc = validate_state(out))) return rc;
+    if (rc || (rc = validate_state(d, in)) || (rc = validate_state(d, out))) return rc;
     return launch_velocity(make_geo(*d), *m, make_shots(*sh), make_state(*in), make_state(*out), it,
                            (hipStream_t)stream);
 }
@@ -173,7 +182,7 @@ extern "C" int efm_record(const EfmDims *d, const EfmShots *sh, const EfmState *
     if (!d || !sh || !gathers) return fail(EFM_E_ARG, "efm_record: null argument");
     int rc = check_dims(d);
     if (rc) return rc;
-    if ((rc = validate_state(st))) return rc;
+    if ((rc = validate_shots(d, sh)) || (rc = validate_state(d, st))) return rc;
     if (it < 0 || it >= d->ntmax) return fail(EFM_E_ARG, "efm_record: it out of range");
     return launch_record(make_geo(*d), make_shots(*sh), make_state(*st), gathers, it,
                          (hipStream_t)stream);
@@ -183,7 +192,7 @@ extern "C" int efm_forward(const EfmDims *d, const EfmMedium *m, const EfmShots 
                            const EfmState *st, float *gathers, float *borders, int W, int it0,
                            int nsteps, void *stream) {
     int rc = validate(d, m, sh);
-    if (rc || (rc = validate_state(st))) return rc;
+    if (rc || (rc = validate_state(d, st))) return rc;
     if (it0 < 0 || nsteps < 0 || it0 + nsteps > d->ntmax)
         return fail(EFM_E_ARG, "efm_forward: step range outside [0, ntmax)");
     if (borders && (W < 0 || W + H + 1 > d->nx / 2 || W + H + 1 > d->ny / 2 || W + H + 1 > d->nz / 2))
@@ -214,7 +223,7 @@ extern "C" int efm_forward_graph(const EfmDims *d, const EfmMedium *m, const Efm
                                  const EfmState *st, float *gathers, int it0, int nsteps, int chunk,
                                  int *counter, void *stream) {
     int rc = validate(d, m, sh);
-    if (rc || (rc = validate_state(st))) return rc;
+    if (rc || (rc = validate_state(d, st))) return rc;
     if (it0 < 0 || nsteps < 0 || it0 + nsteps > d->ntmax)
         return fail(EFM_E_ARG, "efm_forward_graph: step range outside [0, ntmax)");
     if (chunk < 1 || !counter) return fail(EFM_E_ARG, "efm_forward_graph: chunk >= 1 and a counter are required");
@@ -274,7 +283,7 @@ extern "C" int efm_forward_states(const EfmDims *d, const EfmMedium *m, const Ef
     if (it0 < 0 || nsteps < 0 || it0 + nsteps > d->ntmax)
         return fail(EFM_E_ARG, "efm_forward_states: step range outside [0, ntmax)");
     for (int k = 0; k <= nsteps; ++k)
-        if ((rc = validate_state(states[k]))) return rc;
+        if ((rc = validate_state(d, states[k]))) return rc;
     const Geo g = make_geo(*d);
     const ShotPtrs shp = make_shots(*sh);
     hipStream_t hs = (hipStream_t)stream;

@@ -129,6 +129,12 @@ int check_launch(const char *what);
 // Grid, CPML widths, record length and the size limits of the sweep kernels
 // (32-bit offsets inside a shot, one dispatch of < 2^32 work-items per sweep).
 int check_dims(const EfmDims *d);
+// Null checks of the C-ABI structs (EFM_E_ARG with a message): the shot
+// description's arrays, dims + medium + shots, and one state (ψ slab pointers
+// are required for every axis that has a CPML layer).
+int validate_shots(const EfmDims *d, const EfmShots *sh);
+int validate(const EfmDims *d, const EfmMedium *m, const EfmShots *sh);
+int validate_state(const EfmDims *d, const EfmState *s);
 
 // ---- tile configuration chosen by the autotuner ------------------------------
 struct Tile {

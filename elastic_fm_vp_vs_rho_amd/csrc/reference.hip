@@ -135,8 +135,7 @@ using namespace efm;
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 static int check_common(const EfmDims *d, const EfmMedium *m, const EfmShots *sh) {
-    if (!d || !m || !sh) return fail(EFM_E_ARG, "null argument");
-    return check_dims(d);
+    return validate(d, m, sh);
 }
 
 extern "C" int efm_inject_residual(const EfmDims *d, const EfmShots *sh, const EfmState *st,
@@ -144,6 +143,8 @@ extern "C" int efm_inject_residual(const EfmDims *d, const EfmShots *sh, const E
     if (!d || !sh || !st || (!res && d->nrec > 0))
         return fail(EFM_E_ARG, "efm_inject_residual: null argument");
     if (int rc = check_dims(d)) return rc;
+    if (int rc = validate_shots(d, sh)) return rc;
+    if (int rc = validate_state(d, st)) return rc;
     const Geo g = make_geo(*d);
     if (g.nrec <= 0) return EFM_OK;
     hipLaunchKernelGGL(k_inject_ref, dim3(g.S), dim3(64), 0, (hipStream_t)stream, g,
@@ -156,6 +157,7 @@ extern "C" int efm_image_mt_ref(const EfmDims *d, const EfmMedium *m, const EfmS
     int rc = check_common(d, m, sh);
     if (rc) return rc;
     if (!adj || !gm6) return fail(EFM_E_ARG, "efm_image_mt_ref: null argument");
+    if ((rc = validate_state(d, adj))) return rc;
     const Geo g = make_geo(*d);
     hipLaunchKernelGGL(k_image_mt_ref, dim3(nblk(g.S, 64)), dim3(64), 0, (hipStream_t)stream, g,
                        m->lam, m->mu, make_shots(*sh), make_state(*adj), gm6, it);
@@ -169,6 +171,7 @@ extern "C" int efm_backward_mt_ref(const EfmDims *d, const EfmMedium *m, const E
     if (rc) return rc;
     if (!adj || (!res && d->nrec > 0) || !gm6)
         return fail(EFM_E_ARG, "efm_backward_mt_ref: null argument");
+    if ((rc = validate_state(d, adj))) return rc;
     if (nsteps < 0 || nsteps > d->ntmax) return fail(EFM_E_ARG, "efm_backward_mt_ref: nsteps out of range");
     Geo g = make_geo(*d);
     if (g.c < 0) g.c = -g.c;  // the adjoint runs the forward operator with +dt (M:2066-2068)
@@ -203,6 +206,7 @@ extern "C" int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, con
     if (rc) return rc;
     if (!fwd || !vpres || !borders || !adj || (!res && d->nrec > 0) || !glam || !gmu || !grho)
         return fail(EFM_E_ARG, "efm_backward_remake_ref: null argument");
+    if ((rc = validate_state(d, fwd)) || (rc = validate_state(d, adj))) return rc;
     if (W + H + 1 > d->nx / 2 || W + H + 1 > d->ny / 2 || W + H + 1 > d->nz / 2)
         return fail(EFM_E_ARG, "efm_backward_remake_ref: CPML too wide for the grid");
     Geo ga = make_geo(*d);

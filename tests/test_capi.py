@@ -137,3 +137,55 @@ def test_bench_traffic_table_covers_every_compiled_tile(L):
     for k in autotune.KERNELS:
         assert set(bench.MEASURED_TRAFFIC[k]) == keys, k
         assert all(v > 0 for v in bench.MEASURED_TRAFFIC[k].values())
+
+
+def _structs(nrec=4):
+    from elastic_fm_vp_vs_rho_amd._capi import EfmDims, EfmMedium, EfmShots, EfmState
+    d = EfmDims(16, 16, 16, 1, 3, 3, 3, 3, 0, 3, 10, nrec, 0.1)
+    return d, EfmMedium(1, 1, 1), EfmShots(*([1] * 12)), EfmState(*([1] * 12))
+
+
+@pytest.mark.parametrize("hole", ["ax", "nt", "rec", "wavelet", "src"])
+def test_null_shot_arrays_are_refused_before_launch(L, hole):
+    """Every entry point that reads the shot description checks its arrays
+    (a null one would reach a kernel and fault the GPU)."""
+    from elastic_fm_vp_vs_rho_amd import _capi
+    d, m, sh, st = _structs()
+    setattr(sh, hole, None)
+    by = ctypes.byref
+    calls = {
+        "efm_forward": lambda: L.efm_forward(by(d), by(m), by(sh), by(st), 1, None, 0, 0, 1, None),
+        "efm_record": lambda: L.efm_record(by(d), by(sh), by(st), 1, 0, None),
+        "efm_inject_residual": lambda: L.efm_inject_residual(by(d), by(sh), by(st), 1, 0, None),
+        "efm_image_mt_ref": lambda: L.efm_image_mt_ref(by(d), by(m), by(sh), by(st), 1, 0, None),
+        "efm_backward_mt_ref": lambda: L.efm_backward_mt_ref(by(d), by(m), by(sh), by(st), 1, 1, 1, None),
+        "efm_backward_remake_ref": lambda: L.efm_backward_remake_ref(
+            by(d), by(m), by(sh), by(st), 1, 1, 3, by(st), 1, 1.0, 1.0, 1.0, 1.0, 1, 1, 1, None),
+    }
+    fwd = (ctypes.POINTER(type(st)) * 2)(ctypes.pointer(st), ctypes.pointer(st))
+    calls["efm_adjoint"] = lambda: L.efm_adjoint(by(d), by(m), 1, 1, by(sh), fwd, by(st), by(st), 1, 1, 1,
+                                                 0, 1, 1, None)
+    for name, call in calls.items():
+        assert call() == -1, name
+        assert "null" in _capi.last_error(), (name, _capi.last_error())
+
+
+@pytest.mark.parametrize("hole", ["fwd", "adj_field", "adj_psi", "adj2_psi"])
+def test_efm_adjoint_checks_every_state(L, hole):
+    from elastic_fm_vp_vs_rho_amd import _capi
+    from elastic_fm_vp_vs_rho_amd._capi import EfmState
+    d, m, sh, st = _structs()
+    good = [EfmState(*([1] * 12)) for _ in range(4)]
+    adj, adj2 = EfmState(*([1] * 12)), EfmState(*([0] * 9 + [1] * 3))   # adj2: ψ buffers only
+    if hole == "fwd":
+        good[2].tzz = None               # one forward state in the middle of the range
+    elif hole == "adj_field":
+        adj.vy = None
+    elif hole == "adj_psi":
+        adj.psi_z = None
+    else:
+        adj2.psi_x = None
+    fwd = (ctypes.POINTER(EfmState) * 4)(*[ctypes.pointer(s) for s in good])
+    by = ctypes.byref
+    rc = L.efm_adjoint(by(d), by(m), 1, 1, by(sh), fwd, by(adj), by(adj2), 1, 1, 1, 5, 8, 0, None)
+    assert rc == -1 and "null" in _capi.last_error(), _capi.last_error()
