@@ -39,6 +39,10 @@ class EfmState(ctypes.Structure):
                                         "psi_x", "psi_y", "psi_z")]
 
 
+class EfmSmooth(ctypes.Structure):
+    _fields_ = [("sigma", c_float), ("truncate", c_float), ("margin", c_int * 6)]
+
+
 class EfmShots(ctypes.Structure):
     _fields_ = [(k, c_void_p) for k in ("ax", "bx", "ay", "by", "az", "bz", "src", "m6",
                                         "wavelet", "nt", "rec", "recmask")]
@@ -81,6 +85,12 @@ _SIGS = {
                                  c_void_p, c_int, P(EfmState), c_void_p, c_float, c_float, c_float,
                                  c_float, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "efm_border_floats": ([P(EfmDims), c_int], c_ll),
+    "efm_condition_gradient": ([P(EfmSmooth), c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                c_void_p], c_int),
+    "efm_model_update": ([c_void_p, c_void_p, ctypes.c_double, c_void_p, c_void_p, ctypes.c_double,
+                          c_void_p, c_void_p] + [c_int] * 4 + [c_void_p] * 6, c_int),
+    "efm_model_rho_nafe_drake": ([c_void_p, c_void_p, c_ll, c_void_p], c_int),
+    "efm_volume_axpy": ([c_float, c_void_p, c_void_p, c_ll, c_void_p], c_int),
     # initializer network (include/efm_initnet.h)
     "efm_net_cast_bf16": ([c_void_p, c_void_p, c_ll, c_void_p], c_int),
     "efm_net_gemm": ([c_int, c_int, c_int, c_void_p, c_ll, c_ll, c_void_p, c_ll, c_ll, c_void_p, c_ll,

@@ -25,11 +25,21 @@ import os
 import numpy as np
 import torch
 
-from . import _capi, model as _M
+from . import _capi, conditioning as _C, model as _M
 from .lbfgs import L_BFGS, compute_sk_yk
 from .propagator import Grid, Propagator, ShotBatch, Wavefield, _ptr, _stream
 
 H_STENCIL = 2
+
+
+def steepest_alphas(lam, mu, rho, tl, tm, tr):
+    """Steps of the fixed-α iterations of fwi_Vp_Vs_Rho (M:2669-2671): each block
+    moves by a set fraction of its model's 2-norm at the image's maximum,
+    α_λ = 1e-5·‖λ‖₂/max(tλ), α_μ = 1e-4·‖μ‖₂/max(tμ), α_ρ = 1e-6·‖ρ‖₂/max(tρ)
+    (α_ρ is reported but ρ follows Vp, M:2679).  Returns (α_λ, α_μ, α_ρ)."""
+    def a(frac, model, image):
+        return float(frac * np.sqrt(np.sum(np.ravel(model).astype(np.float64) ** 2)) / np.amax(image))
+    return a(1e-5, lam, tl), a(1e-4, mu, tm), a(1e-6, rho, tr)
 
 
 def _device(device=None):
@@ -283,6 +293,8 @@ class Elastic3D:
         self.grid = Grid(self.nx, self.ny, self.nz, self.dx, ondicula.dt,
                          (W, W, W, W, 0 if pml.free_top else W, W))
         self.M = StressTensor(0, 0, 0, 0, 0, 0)
+        self.iteration = 0
+        self._gk_device = None
         self.forwFields = None
         self._borders = None
         self._shots = None
@@ -410,6 +422,7 @@ class Elastic3D:
             float(self.ondicula.dt), float(self.dx), float(self.dy), float(self.dz), _ptr(glam), _ptr(gmu),
             _ptr(grho), _stream()), "efm_backward_remake_ref")
         self.forwFields = None          # consumed by the reverse-time reconstruction
+        self._gk_device = (glam, gmu, grho)
         self.gk.Lambda = glam.cpu().numpy()
         self.gk.Mu = gmu.cpu().numpy()
         self.gk.Rho = grho.cpu().numpy()
@@ -483,10 +496,16 @@ class Elastic3D:
         return hist
 
     # -- Vp/Vs/ρ FWI (M:2349-2719) -------------------------------------------------
+    def _accumulate(self, acc, g, sign=1.0):
+        """acc += sign·g on the device (the reference's float32 tempLambda += gk.Lambda,
+        M:2466-2468)."""
+        _C.axpy(sign, g, acc)
+
     def _all_shots_gradient(self, data_obs, listaTM, gradient):
-        """Sum over earthquakes of the misfit and the λ/μ/ρ descent images."""
-        tl = np.zeros((self.ny, self.nz, self.nx), np.float64)
-        tm, tr = np.zeros_like(tl), np.zeros_like(tl)
+        """Sum over earthquakes of the misfit and the λ/μ/ρ descent images
+        (device volumes, float32 sums like the reference's temp* arrays)."""
+        z = lambda: torch.zeros(self.grid.shape, dtype=torch.float32, device=self.device)
+        tl, tm, tr = z(), z(), z()
         total = 0.0
         modData = [None] * self.geometria.getLenSismos()
         for i in range(self.geometria.getLenSismos()):
@@ -498,47 +517,49 @@ class Elastic3D:
                 J, gr = self.gradient_exact(self.obsGather)
                 total += J
                 # the reference's update adds its image (a descent direction)
-                tl -= gr["lam"].double().cpu().numpy()
-                tm -= gr["mu"].double().cpu().numpy()
-                tr -= gr["rho"].double().cpu().numpy()
+                self._accumulate(tl, gr["lam"], -1.0)
+                self._accumulate(tm, gr["mu"], -1.0)
+                self._accumulate(tr, gr["rho"], -1.0)
                 host = gr["gathers"][0].cpu().numpy()
                 d = Gather(self.nt, self.LenStations)
                 d.Vx, d.Vy, d.Vz = host[0], host[1], host[2]
                 modData[i] = d
+                if self.iteration == 0:
+                    self._save("Gathers/modData0", d)      # every shot, one name (M:2414-2417)
                 continue
             self.forwardFields("mod")
             d = Gather(self.nt, self.LenStations)
             d.Vx, d.Vy, d.Vz = self.getGathers()
             modData[i] = d
+            if self.iteration == 0:
+                self._save("Gathers/modData0", d)          # every shot, one name (M:2414-2417)
             self.computeResidual()
             r = self.resGather
             total += 0.5 * (np.sum(r.Vx ** 2) + np.sum(r.Vy ** 2) + np.sum(r.Vz ** 2))
             self.backwardFields_FowardFieldsRemake()
-            tl += self.gk.Lambda; tm += self.gk.Mu; tr += self.gk.Rho
+            gl, gm, gr_ = self._gk_device
+            self._accumulate(tl, gl); self._accumulate(tm, gm); self._accumulate(tr, gr_)
         return total, tl, tm, tr, modData
 
     def _condition(self, g, sigma=1.0, top=10):
-        """Gradient smoothing and masks (M:2479-2504): σ=1 Gaussian (scipy
-        defaults), zero z < 10 and the CPML+2 zone on x, y and the bottom."""
-        from scipy.ndimage import gaussian_filter
-        g = gaussian_filter(g, sigma=sigma)
-        W = self.lenPML
-        g[:, :top, :] = 0
-        g[:W + 2, :, :] = 0
-        g[self.ny - W - 2:, :, :] = 0
-        g[:, :, :W + 2] = 0
-        g[:, :, self.nx - W - 2:] = 0
-        g[:, self.nz - W - 2:, :] = 0
-        return g
+        """Gradient smoothing and masks (M:2479-2504) on the device: σ=1 Gaussian
+        (scipy's defaults: mode='reflect', truncate=4), z < 10 and the CPML+2 zone
+        of x, y and the bottom zeroed."""
+        return _C.condition_gradient(g, sigma, 4.0, _C.reference_margins(self.nx, self.ny, self.nz,
+                                                                           self.lenPML, top))
 
-    def _apply_model(self, lam, mu, cRho):
-        self.mk.Lambda = np.asarray(lam, np.float32)
-        self.mk.Mu = np.asarray(mu, np.float32)
-        self.mk.compute_Vp_Vs()
-        self.mk.compute_Rho()
-        self.mk.Rho = np.asarray(self.mk.Rho, np.float32)
-        self.mk.Rho[:, :10, :] = cRho[:, :10, :]
-        self.mk.hostToDevice(self.device)
+    def _apply_model(self, lam0, dlam, alpha_lam, mu0, dmu, alpha_mu, cRho):
+        """λ = λ0 + α_λ·dλ, μ = μ0 + α_μ·dμ, then compute_Vp_Vs with the current ρ,
+        compute_Rho and ρ restored in the top 10 cells (M:2577-2583, 2688-2694),
+        on the device; the host copies follow."""
+        lam, mu, vp, vs, rho = _C.model_update(lam0, mu0, self.mk.d_Rho, dlam, alpha_lam, dmu, alpha_mu,
+                                               rho_keep=cRho, top=10)
+        self._set_model_device(lam, mu, vp, vs, rho)
+
+    def _set_model_device(self, lam, mu, vp, vs, rho):
+        mk = self.mk
+        mk.d_Lambda, mk.d_Mu, mk.d_Vp, mk.d_Vs, mk.d_Rho = lam, mu, vp, vs, rho
+        mk.Lambda, mk.Mu, mk.Vp, mk.Vs, mk.Rho = (t.cpu().numpy() for t in (lam, mu, vp, vs, rho))
 
     def _misfit_all(self, data_obs, listaTM):
         total = 0.0
@@ -559,47 +580,52 @@ class Elastic3D:
         masks, steepest steps for iter < ite_cte, then L-BFGS with α halving
         from 1 (at most nTry trials); ρ follows Vp through Nafe-Drake.
         ``gradient='exact'`` replaces the reference image by the exact
-        (negated) gradient of the same misfit."""
+        (negated) gradient of the same misfit.  Images, conditioning and model
+        maps stay on the device; the L-BFGS history is lbfgs.py's host arrays,
+        as in the reference."""
         self.costFunction = []
         shape = (lbfgsLayers, self.ny, self.nz, self.nx)
         skL, ykL = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
         skM, ykM = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
         skR, ykR = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
         for it in range(nIter):
-            total, tl, tm, tr, modData = self._all_shots_gradient(data_obs, listaTM, gradient)
-            tr = self._condition(tr).astype(np.float32)
-            tm = self._condition(tm).astype(np.float32)
-            tl = self._condition(tl).astype(np.float32)
+            self.iteration = it
+            total, tl_d, tm_d, tr_d, modData = self._all_shots_gradient(data_obs, listaTM, gradient)
+            tr_d, tm_d, tl_d = self._condition(tr_d), self._condition(tm_d), self._condition(tl_d)
+            tl, tm, tr = (np.asarray(t.cpu().numpy(), np.float32) for t in (tl_d, tm_d, tr_d))
+            self._save_gathers_fq(modData)
             self.costFunction.append(float(total))
             compute_sk_yk(skL, ykL, self.mk.Lambda.copy(), tl, lbfgsLayers, it)
             compute_sk_yk(skM, ykM, self.mk.Mu.copy(), tm, lbfgsLayers, it)
             compute_sk_yk(skR, ykR, self.mk.Rho.copy(), tr, lbfgsLayers, it)
-            cL, cM, cR = self.mk.Lambda.copy(), self.mk.Mu.copy(), self.mk.Rho.copy()
+            mk = self.mk
+            cL, cM, cR = mk.d_Lambda.clone(), mk.d_Mu.clone(), mk.d_Rho.clone()
+            saved = (cL, cM, mk.d_Vp.clone(), mk.d_Vs.clone(), cR)
             if it >= ite_cte and it > 0:
-                rL = L_BFGS(skL, ykL, tl, lbfgsLayers, it)
-                rM = L_BFGS(skM, ykM, tm, lbfgsLayers, it)
+                rL = torch.as_tensor(np.ascontiguousarray(L_BFGS(skL, ykL, tl, lbfgsLayers, it), np.float32),
+                                     device=self.device)
+                rM = torch.as_tensor(np.ascontiguousarray(L_BFGS(skM, ykM, tm, lbfgsLayers, it), np.float32),
+                                     device=self.device)
                 # halving line search, control flow of M:2547-2664: a NaN trial or the
                 # nTry-th trial (whatever its misfit) restores the model and stops
                 alpha, tries, best = 1.0, 0, 1e20
                 while best > self.costFunction[it] or np.isnan(best):
                     tries += 1
                     alpha /= 2
-                    self._apply_model(cL - alpha * rL, cM - alpha * rM, cR)
+                    self._apply_model(cL, rL, -alpha, cM, rM, -alpha, cR)
                     best = self._misfit_all(data_obs, listaTM)
                     if np.isnan(best) or tries >= nTry:
-                        self.mk.Lambda, self.mk.Mu, self.mk.Rho = cL, cM, cR
-                        self.mk.hostToDevice(self.device)
+                        self._set_model_device(*saved)
                         if np.isnan(best) and tries < nTry:
                             continue
                         break
                 else:
-                    self._apply_model(cL - alpha * rL, cM - alpha * rM, cR)
+                    self._apply_model(cL, rL, -alpha, cM, rM, -alpha, cR)
                     self.costFunction[it] = best
             else:
-                self.alpha.Rho = 1e-6 * np.sqrt(np.sum(np.ravel(self.mk.Rho) ** 2)) / np.amax(tr)
-                self.alpha.Mu = 1e-4 * np.sqrt(np.sum(np.ravel(self.mk.Mu) ** 2)) / np.amax(tm)
-                self.alpha.Lambda = 1e-5 * np.sqrt(np.sum(np.ravel(self.mk.Lambda) ** 2)) / np.amax(tl)
-                self._apply_model(self.mk.Lambda + self.alpha.Lambda * tl, self.mk.Mu + self.alpha.Mu * tm, cR)
+                self.alpha.Lambda, self.alpha.Mu, self.alpha.Rho = steepest_alphas(
+                    self.mk.Lambda, self.mk.Mu, self.mk.Rho, tl, tm, tr)
+                self._apply_model(mk.d_Lambda, tl_d, self.alpha.Lambda, mk.d_Mu, tm_d, self.alpha.Mu, cR)
             self._save_fwi(tl, tm, tr)
         return self.costFunction
 
@@ -622,6 +648,18 @@ class Elastic3D:
             np.save(os.path.join(self.results_dir, f"TM/Mmod{k}{i}.npy"), v)
         self._save(f"Gathers/modData{i}", modData)
         self._save(f"Gathers/resData{i}", self.resGather)
+
+    def _save_gathers_fq(self, modData):
+        """Gathers/modData_{fq}Hz.npy (M:2523): the list of per-earthquake Gather
+        objects of this iteration, saved as a numpy object array (pickled, as
+        np.save does for the reference's list; read back with allow_pickle=True)."""
+        if not self.results_dir:
+            return
+        os.makedirs(os.path.join(self.results_dir, "Gathers"), exist_ok=True)
+        arr = np.empty(len(modData), dtype=object)
+        arr[:] = modData
+        np.save(os.path.join(self.results_dir, f"Gathers/modData_{self.ondicula.fq}Hz.npy"), arr,
+                allow_pickle=True)
 
     def _save_fwi(self, tl, tm, tr):
         if not self.results_dir:

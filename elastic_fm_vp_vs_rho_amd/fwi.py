@@ -21,7 +21,7 @@ import time
 import numpy as np
 import torch
 
-from . import autotune, lbfgs, workloads
+from . import autotune, conditioning as _C, lbfgs, workloads
 from .propagator import Grid, Propagator, ShotBatch, available_bytes, lame, lame_adjoint
 
 
@@ -146,16 +146,25 @@ def update_mask(grid: Grid, margin=2):
 
 
 def invert(survey: Survey, vp0, vs0, rho0, m60, obs, iters, batch, layers=7,
-           step=(20.0, 12.0, 10.0, 0.05), tries=6, log=None):
+           step=(20.0, 12.0, 10.0, 0.05), tries=6, log=None, smooth=0.0):
     """Joint L-BFGS on (Vp, Vs, ρ, MT).  The search variable z maps to the model
     as x = x0 + P z with a fixed diagonal P per block, set at the first gradient
     so that the first steepest step (α = 1) changes Vp, Vs, ρ at most by
     `step[0:3]` (m/s, m/s, kg/m³) and the MTs by `step[3]` of their mean
     Frobenius norm.  The line search
     halves from α = 1 until the misfit decreases (at most `tries` trials).
-    `layers` is lbfgs.py's history size (layers - 2 pairs are used)."""
+    `layers` is lbfgs.py's history size (layers - 2 pairs are used).
+    The Vp, Vs, ρ gradients are conditioned on the device (efm_condition_gradient):
+    zeroed in the CPML + 2 cells of every face and, when `smooth` > 0, smoothed
+    with a Gaussian of σ = `smooth` cells (the reference smooths with σ = 1,
+    M:2479-2482)."""
     dev = survey.dev
     mask = update_mask(survey.grid).to(dev)
+    w = survey.grid.widths
+    margins = tuple(v + 2 for v in w)
+
+    def condition(gv):
+        return _C.condition_gradient(gv, smooth, 4.0, margins)
     nv = vp0.numel()
     x0 = [vp0, vs0, rho0]
     m60 = np.asarray(m60, np.float64)
@@ -179,22 +188,24 @@ def invert(survey: Survey, vp0, vs0, rho0, m60, obs, iters, batch, layers=7,
             J, gvp, gvs, grho, gm6 = gradient(survey, vp0, vs0, rho0, m60, obs, batch)
             # a steepest step of unit length moves x by P²·g: size P per block so
             # that it moves each block by its `step`
+            gvp, gvs, grho = condition(gvp), condition(gvs), condition(grho)
             fro = np.sqrt((m60[:, :3] ** 2).sum(1) + 2 * (m60[:, 3:] ** 2).sum(1)).mean()
-            P = [math.sqrt(step[0] / max(float((gvp * mask).abs().max()), 1e-30)),
-                 math.sqrt(step[1] / max(float((gvs * mask).abs().max()), 1e-30)),
-                 math.sqrt(step[2] / max(float((grho * mask).abs().max()), 1e-30)),
+            P = [math.sqrt(step[0] / max(float(gvp.abs().max()), 1e-30)),
+                 math.sqrt(step[1] / max(float(gvs.abs().max()), 1e-30)),
+                 math.sqrt(step[2] / max(float(grho.abs().max()), 1e-30)),
                  math.sqrt(step[3] * fro / max(float(np.abs(gm6).max()), 1e-30))]
         else:
             vp, vs, rho, m6 = model(z)
             J, gvp, gvs, grho, gm6 = gradient(survey, vp, vs, rho, m6, obs, batch)
+            gvp, gvs, grho = condition(gvp), condition(gvs), condition(grho)
         hist.append(J)
         if log:
             log(f"[fwi] iteration {it}: J = {J:.6e}")
         if it == iters:
             break
-        g = np.concatenate([(float(P[0]) * (gvp * mask)).double().cpu().numpy().ravel(),
-                            (float(P[1]) * (gvs * mask)).double().cpu().numpy().ravel(),
-                            (float(P[2]) * (grho * mask)).double().cpu().numpy().ravel(),
+        g = np.concatenate([float(P[0]) * gvp.double().cpu().numpy().ravel(),
+                            float(P[1]) * gvs.double().cpu().numpy().ravel(),
+                            float(P[2]) * grho.double().cpu().numpy().ravel(),
                             P[3] * gm6.ravel()])
         lbfgs.compute_sk_yk(sk, yk, z, g, layers, it)
         p = -(g if it == 0 else lbfgs.L_BFGS(sk, yk, g, layers, it))

@@ -201,8 +201,9 @@ def c5(n=96, S=20000, nt=1024, nrec=64, f0=10.0, seed=5):
 
 def c4(nx=200, ny=200, nz=100, S=16, nt=2500, nrec=64, f0=10.0, anomalies=True, seed=6):
     """C4 FWI + MT: smoothly layered model (Vp 2600 -> 4200 m/s with depth,
-    Vs = Vp/1.73, Gardner ρ) with two Gaussian anomalies (+8 % and -8 % on Vp
-    and Vs; σ = 8 cells) in the true model; S sources at 0.3-0.7 of the depth
+    Vs = Vp/1.73, Gardner ρ = 310·Vp^0.25) with two Gaussian anomalies (+8 % and
+    -8 % on Vp and Vs, so ±2 % on ρ through Gardner; σ = 8 cells) in the true
+    model; S sources at 0.3-0.7 of the depth
     with normalised MT (seed 6), 10 Hz Ricker, nt steps, 8×8 receivers one
     level below the top CPML; 10-cell CPML on all faces."""
     dx, W = 10.0, 10
@@ -215,7 +216,7 @@ def c4(nx=200, ny=200, nz=100, S=16, nt=2500, nrec=64, f0=10.0, anomalies=True, 
         for (cx, cy, cz, amp) in ((0.35 * nx, 0.4 * ny, 0.55 * nz, 0.08), (0.65 * nx, 0.6 * ny, 0.45 * nz, -0.08)):
             vp *= 1 + amp * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / (2 * 8.0 ** 2))
     vs = vp / 1.73
-    rho = 310.0 * np.broadcast_to(vp_z[None, :, None], (ny, nz, nx)) ** 0.25
+    rho = 310.0 * vp ** 0.25
     vmax = 4200.0 * 1.1
     dt = courant_dt(dx, vmax)
     mx, my = max(2, min(10, (nx - 2 * W) // 4)), max(2, min(10, (ny - 2 * W) // 4))

@@ -213,6 +213,38 @@ int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, const EfmShots
 /* Size in floats of the boundary buffer efm_forward writes for one shot. */
 long long efm_border_floats(const EfmDims *d, int W);
 
+/* ---- FWI gradient conditioning and model maps (fwi_Vp_Vs_Rho, M:2479-2504,
+ * 2577-2583, 2656-2661, 2688-2694; Model M:1200-1206) -------------------- */
+typedef struct EfmSmooth {
+    float sigma;       /* Gaussian σ in cells; 0 = no smoothing (masks only) */
+    float truncate;    /* kernel radius = int(truncate·σ + 0.5) cells, at most 32 (scipy's rule) */
+    int margin[6];     /* cells zeroed from the x_lo, x_hi, y_lo, y_hi, z_lo, z_hi faces */
+} EfmSmooth;
+/* out = mask(gaussian_filter(in, σ, mode='reflect', truncate)) of one
+ * (ny, nz, nx) volume: scipy.ndimage.gaussian_filter's separable passes in its
+ * axis order, each summed in double and rounded to float32, then the face
+ * margins zeroed (the reference zeroes z < 10 and the CPML+2 zone of the other
+ * five faces, M:2479-2504).  work is a caller volume of the same size, needed
+ * when σ > 0; in, out and work must not overlap. */
+int efm_condition_gradient(const EfmSmooth *p, int nx, int ny, int nz, const float *in,
+                           float *out, float *work, void *stream);
+/* The FWI model update of one (ny, nz, nx) model (M:2577-2583, 2688-2694):
+ * lam = lam0 + alpha_lam·dlam and mu = mu0 + alpha_mu·dmu (in double, rounded once;
+ * a null direction keeps lam0 / mu0), then Model.compute_Vp_Vs with rho_in
+ * (vp = sqrt((lam + 2mu)/rho_in), vs = sqrt(mu/rho_in), float32, M:1200-1202)
+ * and, when rho is not null, Model.compute_Rho (Nafe-Drake of vp, M:1204-1206)
+ * except at z < top, where rho_keep (if not null) is copied.  Outputs may
+ * alias the lam0/mu0 inputs; rho must not alias rho_in or rho_keep. */
+int efm_model_update(const float *lam0, const float *dlam, double alpha_lam, const float *mu0,
+                     const float *dmu, double alpha_mu, const float *rho_in, const float *rho_keep, int top,
+                     int nx, int ny, int nz, float *lam, float *mu, float *vp, float *vs,
+                     float *rho, void *stream);
+/* Model.compute_Rho (M:1204-1206): rho = Nafe-Drake(vp), n values. */
+int efm_model_rho_nafe_drake(const float *vp, float *rho, long long n, void *stream);
+/* y += a·x over n floats (per-shot gradient sums, as the reference's
+ * float32 tempLambda += gk.Lambda, M:2466-2468). */
+int efm_volume_axpy(float a, const float *x, float *y, long long n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,106 @@
+"""Device gradient conditioning and model maps against the library routines the
+reference calls on the host: scipy.ndimage.gaussian_filter (σ=1; defaults
+mode='reflect', truncate=4) plus the masks of M:2479-2504, and the numpy model
+maps of Model.compute_Vp_Vs / compute_Rho (M:1200-1206) with the FWI update of
+M:2577-2583 / 2688-2694.  Through the C ABI (conditioning.py is a thin ctypes
+wrapper)."""
+import numpy as np
+import pytest
+import torch
+from scipy.ndimage import gaussian_filter
+
+from elastic_fm_vp_vs_rho_amd import conditioning as C
+from elastic_fm_vp_vs_rho_amd import model as Mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+
+
+def _ref_condition(g, sigma, truncate, margins):
+    out = gaussian_filter(np.asarray(g, np.float32), sigma=sigma, mode="reflect", truncate=truncate)
+    xl, xh, yl, yh, zl, zh = margins
+    ny, nz, nx = out.shape
+    out[:yl] = 0; out[ny - yh:] = 0
+    out[:, :zl] = 0; out[:, nz - zh:] = 0
+    out[:, :, :xl] = 0; out[:, :, nx - xh:] = 0
+    return out
+
+
+@pytest.mark.parametrize("shape,sigma,truncate,margins", [
+    ((37, 41, 29), 1.0, 4.0, (12, 12, 12, 12, 10, 12)),     # the reference's rule, W = 10
+    ((20, 24, 70), 1.0, 4.0, (0, 0, 0, 0, 0, 0)),
+    ((5, 7, 30), 2.5, 4.0, (1, 2, 0, 1, 3, 0)),             # radius 10 > 5, 7: repeated reflections
+    ((16, 9, 33), 0.0, 4.0, (4, 4, 4, 4, 10, 4)),           # masks only
+])
+def test_condition_matches_scipy(gpu_lib, shape, sigma, truncate, margins):
+    g = np.random.default_rng(1).standard_normal(shape).astype(np.float32) * 1e3
+    ref = _ref_condition(g, sigma, truncate, margins)
+    got = C.condition_gradient(_dev(g), sigma, truncate, margins).cpu().numpy()
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    print(f"condition {shape} sigma={sigma}: max err {err:.2e}, identical {np.mean(got == ref):.4f}")
+    assert err <= 2e-7, err          # double sums rounded once per pass, like scipy's float32 path
+    if sigma == 0:
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_driver_condition_is_the_reference_rule(gpu_lib):
+    """Elastic3D._condition = gaussian_filter(σ=1) then z < 10 and the CPML+2 zones zeroed."""
+    from tests.test_driver_contract import _e3d, _host_condition
+    e3 = _e3d(mode=1, device="cuda")
+    g = np.random.default_rng(2).standard_normal((e3.ny, e3.nz, e3.nx)).astype(np.float32)
+    got = e3._condition(_dev(g)).cpu().numpy()
+    ref = _host_condition(e3, g)
+    W = e3.lenPML
+    keep = np.zeros(ref.shape, bool)
+    keep[W + 2:e3.ny - W - 2, 10:e3.nz - W - 2, W + 2:e3.nx - W - 2] = True
+    assert not np.any(got[~keep])
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-7 * np.abs(ref).max())
+
+
+def test_model_update_matches_numpy(gpu_lib):
+    rng = np.random.default_rng(3)
+    shape = (11, 23, 17)
+    vp = (2500 + 1500 * rng.random(shape)).astype(np.float32)
+    vs = (vp / (1.65 + 0.2 * rng.random(shape))).astype(np.float32)
+    rho = Mo.rho_nafe_drake(vp).astype(np.float32)
+    lam, mu = Mo.lame_from_velocity(vp, vs, rho)
+    dl = (rng.standard_normal(shape) * 1e7).astype(np.float32)
+    dm = (rng.standard_normal(shape) * 1e7).astype(np.float32)
+    keep = (rho * 1.01).astype(np.float32)
+    al, am, top = 0.37, -0.8, 10
+    L, M, Vp, Vs, R = (t.cpu().numpy() for t in C.model_update(
+        _dev(lam), _dev(mu), _dev(rho), _dev(dl), al, _dev(dm), am, rho_keep=_dev(keep), top=top))
+    # λ, μ: one rounding of the double update (the reference's float32 model + float64 step)
+    eL = (lam.astype(np.float64) + al * dl.astype(np.float64)).astype(np.float32)
+    eM = (mu.astype(np.float64) + am * dm.astype(np.float64)).astype(np.float32)
+    np.testing.assert_array_equal(L, eL)
+    np.testing.assert_array_equal(M, eM)
+    # compute_Vp_Vs in float32 with the ρ the model held (correctly rounded / and sqrt)
+    eVp, eVs = Mo.velocity_from_lame(eL, eM, rho)
+    np.testing.assert_array_equal(Vp, eVp)
+    np.testing.assert_array_equal(Vs, eVs)
+    # compute_Rho below the top cells, ρ kept above
+    np.testing.assert_allclose(R[:, top:], Mo.rho_nafe_drake(eVp)[:, top:], rtol=1e-6)
+    np.testing.assert_array_equal(R[:, :top], keep[:, :top])
+    # a null direction keeps the model; ρ left out on request
+    L2, M2, _, _, R2 = C.model_update(_dev(lam), _dev(mu), _dev(rho), with_rho=False)
+    np.testing.assert_array_equal(L2.cpu().numpy(), lam)
+    assert R2 is None
+
+
+def test_rho_nafe_drake_and_axpy(gpu_lib):
+    rng = np.random.default_rng(4)
+    vp = (1500 + 5000 * rng.random(10007)).astype(np.float32)
+    got = C.rho_nafe_drake(_dev(vp)).cpu().numpy()
+    np.testing.assert_allclose(got, Mo.rho_nafe_drake(vp), rtol=1e-6)
+    # published point of the polynomial (Brocher 2005, with the reference's 0.4712)
+    assert abs(float(C.rho_nafe_drake(_dev(np.array([3000.0])))[0]) - 2232.0) < 1.0
+    x = rng.standard_normal(5000).astype(np.float32)
+    y = rng.standard_normal(5000).astype(np.float32)
+    yd = _dev(y)
+    C.axpy(-1.0, _dev(x), yd)
+    np.testing.assert_array_equal(yd.cpu().numpy(), y - x)
