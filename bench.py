@@ -1,15 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark: exact FWI gradient (forward + adjoint) of the 128³ three-layer
 model with a Gaussian anomaly, 2000 steps, 64 receivers, one MT source, on one
-MI355X.  A bench step is one full gradient evaluation: forward run keeping what
-the adjoint needs (every state when they fit in HBM, as at 128³; else the last
-states plus √-checkpoints of the first steps, or √nt checkpoints), misfit,
-reverse sweep (adjoint + imaging, re-running any checkpointed segment),
-gradient assembly.
+MI355X.  A bench step is one full gradient evaluation: forward run keeping
+√nt checkpoints of the forward wavefield (not the full history), misfit,
+reverse sweep (each checkpoint segment re-run forward, then adjoint + imaging),
+gradient assembly.  The same gradient with every forward state kept in HBM
+("store") is timed beside it.
 
 Prints one JSON line.  ``value`` is gradient cell-steps per second (grid cells
 × time steps of one gradient, per second of wall time); one gradient cell-step
-is one cell carried through one forward and one adjoint time step.
+is one cell carried through one forward and one adjoint time step.  The line
+also carries the seismogram error of the timed configuration against a
+committed FP64 fixture, the gradient error against the FP64 oracle on a small
+member of the same workload family, and a bounded sample of the C4 gradient.
 """
 from __future__ import annotations
 
@@ -34,6 +37,12 @@ HBM_PEAK_GBS = 8000.0
 #   adj_vel    v̄×3, forward τ×6, b̄×3 read; τ̄×6 and ρ-sums×3 r+w   = 30 words
 #   adj_str    τ̄×6, forward v×3, λ, μ, μ̄×3 read; v̄×3, λμ-sums×5 r+w = 30 words
 ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 120}
+# The byte model of the metric counts an adjoint time step as 120 B (the adjoint
+# state's sweeps) + 24 B (three imaging accumulators read and written) = 144 B
+# per interior cell; the two-sweep design moves 240 B (its τ̄ crosses HBM
+# between the sweeps, it reads the derived medium and keeps 8 accumulators).
+ADJ_STEP_MODEL_BYTES = 144
+FIXTURE_C2 = os.path.join(ROOT, "tests", "golden", "c2_gathers_400.npz")
 
 # HBM bytes per dispatch on the 128^3 C2 grid, by tile "bx,bz,ny,map": rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors that make the counters read
@@ -228,6 +237,83 @@ def cpu_baseline(wl, steps):
     return out
 
 
+def gradient_vs_oracle(n=40, nt=200):
+    """Relative L2 error of the device gradient (λ, μ, ρ per field and the 6 MT
+    components) against the FP64 oracle's exact adjoint, on a small member of the
+    C2 workload family (same layers, source and receiver layout, n³ cells), each
+    with its own misfit residual against the anomaly model's data."""
+    import numpy as np
+    import torch
+    from oracle import elastic_fp64 as O
+    from elastic_fm_vp_vs_rho_amd import workloads
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch, lame
+    wl, wt = workloads.c2(n=n, nt=nt), workloads.c2(n=n, nt=nt, anomaly=True)
+    dev = torch.device("cuda:0")
+    grid = Grid(wl.nx, wl.ny, wl.nz, wl.dx, wl.dt, (wl.W,) * 6)
+    shots = ShotBatch(grid, wl.src, wl.m6, wl.wavelets(), wl.nt, wl.rec, wl.cpml(), device=dev)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    lt, mt = lame(f(wt.vp), f(wt.vs), f(wt.rho))
+    obs, _ = Propagator(grid, lt, mt, f(wt.rho), shots).forward()
+    lam, mu = lame(f(wl.vp), f(wl.vs), f(wl.rho))
+    _, _, glam, gmu, grho, gm6 = Propagator(grid, lam, mu, f(wl.rho), shots).gradient(obs)
+    cp = O.Cpml.from_dict({k: v[0] for k, v in wl.cpml().items()})
+    c = wl.dt / wl.dx
+    w = wl.wavelets()[0]
+    lt_h, mt_h = wt.lame()
+    obs_h, _ = O.forward(O.Medium(lt_h, mt_h, wt.rho), cp, c, tuple(wl.src[0]), wl.m6[0], w, wl.rec[0],
+                         nt, threads=O.default_threads())
+    l_h, m_h = wl.lame()
+    med = O.Medium(l_h, m_h, wl.rho)
+    hist = []
+    g_h, _ = O.forward(med, cp, c, tuple(wl.src[0]), wl.m6[0], w, wl.rec[0], nt,
+                       threads=O.default_threads(), history=hist)
+    r, _ = O.misfit(g_h, obs_h)
+    ref = O.exact_adjoint(med, cp, c, tuple(wl.src[0]), w, wl.rec[0], nt, r, hist)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {"gradient_rel_l2": {"lam": rel(glam.double().cpu().numpy(), ref["lam"]),
+                                "mu": rel(gmu.double().cpu().numpy(), ref["mu"]),
+                                "rho": rel(grho.double().cpu().numpy(), ref["rho"]),
+                                "m6": rel(gm6[0].cpu().numpy(), ref["m6"])},
+            "gradient_sample": f"C2 family at {n}^3 x {nt} steps (same layers, source and receiver "
+                               f"layout), each side with its own residual, vs the FP64 oracle's "
+                               f"exact adjoint"}
+
+
+def c4_gradient_sample(steps):
+    """Seconds per exact gradient of the C4 survey at full size (200x200x100, 16
+    shots, one per sub-batch as the FWI driver runs it) with every record cut to
+    `steps` steps, after one untimed call; tiles tuned for one shot."""
+    import numpy as np
+    import torch
+    from elastic_fm_vp_vs_rho_amd import autotune, fwi, workloads
+    from elastic_fm_vp_vs_rho_amd.propagator import Propagator, lame
+    dev = torch.device("cuda:0")
+    w0 = workloads.c4(nt=steps, anomalies=False)
+    wt = workloads.c4(nt=steps, anomalies=True)
+    survey = fwi.Survey(w0, dev)
+    f = lambda a: fwi._tensor(a, dev)
+    batch = survey.default_batch()
+    saved = [autotune.get_tile(k) for k in range(4)]
+    lam0, mu0 = lame(f(w0.vp), f(w0.vs), f(w0.rho))
+    autotune.tune(Propagator(survey.grid, lam0, mu0, f(w0.rho), survey.batch(np.arange(batch), w0.m6)))
+    obs = fwi.observed(survey, f(wt.vp), f(wt.vs), f(wt.rho), wt.m6, batch)
+    m6 = np.tile([0.5, 0.0, 0.5, 0.0, 0.5, 0.0], (w0.S, 1))
+    args = (survey, f(w0.vp), f(w0.vs), f(w0.rho), m6, obs, batch)
+    fwi.gradient(*args)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwi.gradient(*args)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    strategy = survey.last_strategy
+    survey.release_states()
+    for k, t in enumerate(saved):
+        autotune.set_tile(k, t)
+    return {"sample": f"{w0.name}: 16 shots x {steps} steps (bounded from 2500), full grid",
+            "s_per_gradient": sec, "shots_per_batch": batch, "forward_wavefields": strategy,
+            "gradient_cell_steps_per_s": survey.grid.N * float(w0.nt.sum()) / sec}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,10 +337,15 @@ def main():
     ap.add_argument("--profile-workload", default="c2", choices=("c2", "c3x4", "c4x1"),
                     help="with --profile-kernels: the C2 grid, 4 shots of the 256x256x128 C3 "
                          "grid, or one shot of the 200x200x100 C4 grid")
-    ap.add_argument("--strategy", default="auto", choices=("auto", "store", "hybrid", "checkpoint"),
-                    help="forward wavefields for the adjoint: every state kept in HBM, the last "
-                         "states kept and the first steps checkpointed, or sqrt(nt) checkpoints "
-                         "with segment re-forward")
+    ap.add_argument("--strategy", default="checkpoint", choices=("auto", "store", "hybrid", "checkpoint"),
+                    help="forward wavefields for the adjoint: sqrt(nt) checkpoints with segment "
+                         "re-forward (default), every state kept in HBM, the last states kept and "
+                         "the first steps checkpointed, or whichever fits (auto)")
+    ap.add_argument("--no-store", action="store_true", help="skip timing the store strategy beside it")
+    ap.add_argument("--no-parity", action="store_true", help="skip the errors against FP64")
+    ap.add_argument("--no-c4", action="store_true", help="skip the bounded C4 gradient sample")
+    ap.add_argument("--c4-steps", type=int, default=300,
+                    help="record length of the bounded C4 gradient sample (16 shots, full grid)")
     ap.add_argument("--keep", type=int, default=None,
                     help="steps whose states a hybrid tape keeps (default: as many as fit)")
     args = ap.parse_args()
@@ -339,20 +430,48 @@ def main():
         kt = time_kernels(prop)
         interior = (wl.nx - 4) * (wl.ny - 4) * (wl.nz - 4)
         per = {k: ALG_BYTES[k] * interior / (kt[k] * 1e-3) / 1e9 for k in ("stress", "velocity")}
-        adj_gbs = (ALG_BYTES["adj_velocity"] + ALG_BYTES["adj_stress"]) * interior / (kt["adjoint_step"] * 1e-3) / 1e9
+        adj_gbs = ADJ_STEP_MODEL_BYTES * interior / (kt["adjoint_step"] * 1e-3) / 1e9
+        design = ALG_BYTES["adj_velocity"] + ALG_BYTES["adj_stress"]
         dom = max(kt, key=lambda k: kt[k])
         if dom == "adjoint_step":
-            achieved, name = adj_gbs, "adjoint step (adj_velocity + adj_stress + points)"
-            alg = ALG_BYTES["adj_velocity"] + ALG_BYTES["adj_stress"]
+            achieved, name, alg = adj_gbs, "adjoint step (adj_velocity + adj_stress + points)", ADJ_STEP_MODEL_BYTES
         else:
             achieved, name, alg = per[dom], dom, ALG_BYTES[dom]
+        traffic = measured_traffic(dom, wl)
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, wl),
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                             "kernel": name,
                             "alg_bytes_per_cell": alg, "cells_per_launch": interior,
+                            "traffic_over_alg": traffic / (alg * interior) if traffic else None,
                             "avg_ms": kt[dom], "all_kernels_ms": kt,
                             "all_kernels_gbs": dict(per, adjoint_step=adj_gbs)}
+        if dom == "adjoint_step":
+            line["roofline"]["design_bytes_per_cell"] = design
+            line["roofline"]["design_gbs"] = design * interior / (kt[dom] * 1e-3) / 1e9
+    if not args.no_store and strategy != "store":
+        prop.gradient(obs, K, "store")
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        t1 = time.perf_counter()
+        ns = max(1, min(args.steps, 2))
+        for _ in range(ns):
+            prop.gradient(obs, K, "store")
+        torch.cuda.synchronize()
+        ts = (time.perf_counter() - t1) / ns
+        line["store"] = {"s_per_gradient": ts, "value": N * nt * wl.S / ts,
+                         "peak_hbm_gb": torch.cuda.max_memory_allocated() / 1e9,
+                         "note": "every forward state kept in HBM (no re-forward)"}
+    if not args.no_parity and os.path.exists(FIXTURE_C2):
+        ref = np.load(FIXTURE_C2)["gathers"]
+        n400 = ref.shape[1]
+        got = g[0, :, :n400].double().cpu().numpy()
+        line["err_vs_fp64"] = {
+            "seismogram_rel_l2": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+            "seismogram_sample": f"timed configuration, first {n400} of {nt} steps, all 64 receivers "
+                                 f"x 3 components, vs the committed FP64 oracle fixture"}
     prop.release_states()
+    if not args.no_c4:
+        line["c4_gradient_sample"] = c4_gradient_sample(args.c4_steps)
     if not args.no_batched:
         c5 = workloads.c5(S=64, nt=256)
         c5.nt = np.array([128 + (37 * s) % 129 for s in range(c5.S)])      # uneven, 128..256
@@ -360,6 +479,8 @@ def main():
                                     "C5": batched_vs_plain(c5, 64, int(c5.nt.max()))}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+        if not args.no_parity:
+            line.setdefault("err_vs_fp64", {}).update(gradient_vs_oracle())
     print(json.dumps(line))
 
 

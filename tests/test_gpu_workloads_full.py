@@ -1,0 +1,133 @@
+"""The C2, C4 and C5 workloads at their full grid sizes (record lengths bounded
+so the suite stays within minutes):
+
+* C2 (128³): the first 400 steps of the bench's timed shot against a committed
+  FP64 oracle fixture (the first arrivals reach the receivers after ≈250);
+* C4 (200×200×100, 16 shots): one shot's seismograms over 300 steps against an
+  FP64 fixture, and the 16-shot gradient computed in sub-batches of 4 equal to
+  the one-shot-at-a-time gradient (the reference sums per-shot gradients,
+  M:2383-2470);
+* C5 (96³, 1024 steps): two shots advanced together against FP64 fixtures, and
+  the initializer network at its full specification (192 input channels × 1024
+  samples, 64 channels, 4 blocks) forward and backward against the FP64
+  restatement (oracle/initnet_fp64.py) at batch 4.
+
+Fixtures: tests/golden/make_large_fixtures.py."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tests.problems import rel_l2
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _fixture(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def _prop(wl, idx, steps=None):
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch, lame
+    dev = torch.device("cuda")
+    idx = np.asarray(idx)
+    grid = Grid(wl.nx, wl.ny, wl.nz, wl.dx, wl.dt, (wl.W,) * 6)
+    w = wl.wavelets()[idx]
+    nt = wl.nt[idx]
+    if steps is not None:
+        w, nt = w[:, :steps], np.minimum(nt, steps)
+    sh = ShotBatch(grid, wl.src[idx], wl.m6[idx], w, nt, wl.rec[idx],
+                   {k: v[idx] for k, v in wl.cpml().items()}, device=dev)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+    rho = f(wl.rho)
+    lam, mu = lame(f(wl.vp), f(wl.vs), rho)
+    return Propagator(grid, lam, mu, rho, sh)
+
+
+def test_c2_first_400_steps_against_fp64(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd import workloads
+    ref = _fixture("c2_gathers_400.npz")["gathers"]
+    g, _ = _prop(workloads.c2(), [0], steps=400).forward()
+    e = rel_l2(g[0].double().cpu().numpy(), ref)
+    print(f"C2 128^3, 400 steps: seismogram rel L2 {e:.2e}")
+    assert e < 1e-5, e
+
+
+def test_c4_shot_against_fp64(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd import workloads
+    fx = _fixture("c4_shot0_300.npz")
+    wl = workloads.c4(S=16, nt=300, anomalies=True)
+    assert np.array_equal(wl.src[0], fx["src"])
+    g, _ = _prop(wl, [0]).forward()
+    e = rel_l2(g[0].double().cpu().numpy(), fx["gathers"])
+    print(f"C4 200x200x100 shot 0, 300 steps: seismogram rel L2 {e:.2e}")
+    assert e < 1e-5, e
+
+
+def test_c4_subbatched_gradient_equals_one_shot_at_a_time(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd import fwi, workloads
+    dev = torch.device("cuda")
+    w0 = workloads.c4(nt=200, anomalies=False)
+    wt = workloads.c4(nt=200, anomalies=True)
+    survey = fwi.Survey(w0, dev)
+    f = lambda a: fwi._tensor(a, dev)
+    obs = fwi.observed(survey, f(wt.vp), f(wt.vs), f(wt.rho), wt.m6, 4)
+    m6 = np.tile([0.5, 0.0, 0.5, 0.0, 0.5, 0.0], (w0.S, 1))
+    args = (survey, f(w0.vp), f(w0.vs), f(w0.rho), m6, obs)
+    J1, *g1 = fwi.gradient(*args, 1)
+    survey.release_states()
+    J4, *g4 = fwi.gradient(*args, 4)
+    survey.release_states()
+    print(f"C4 16 shots x 200 steps: J {J1:.6e} (one at a time) vs {J4:.6e} (4 per sub-batch)")
+    assert abs(J4 - J1) <= 1e-12 * J1
+    for a, b, name in zip(g4[:3], g1[:3], ("vp", "vs", "rho")):
+        e = rel_l2(a.double().cpu().numpy(), b.double().cpu().numpy())
+        assert e < 1e-6, (name, e)
+        assert float(b.abs().max()) > 0
+    np.testing.assert_array_equal(g4[3], g1[3])
+
+
+def test_c5_two_shots_against_fp64(gpu_lib):
+    from elastic_fm_vp_vs_rho_amd import workloads
+    fx = _fixture("c5_two_shots.npz")
+    wl = workloads.c5(S=2, nt=1024)
+    assert np.array_equal(wl.src, fx["src"])
+    g, _ = _prop(wl, [0, 1]).forward(graph_chunk=64)        # the batched hipGraph loop
+    for s in range(2):
+        e = rel_l2(g[s].double().cpu().numpy(), fx["gathers"][s])
+        print(f"C5 96^3 shot {s}, 1024 steps: seismogram rel L2 {e:.2e}")
+        assert e < 1e-5, (s, e)
+
+
+def test_c5_network_at_full_spec_against_fp64(gpu_lib):
+    from oracle import initnet_fp64 as R
+    from elastic_fm_vp_vs_rho_amd.initnet import InitNet, NetSpec
+    spec = NetSpec()
+    assert (spec.cin, spec.length, spec.channels, spec.nblocks) == (192, 1024, 64, 4)
+    ospec = R.Spec(**{k: getattr(spec, k) for k in ("cin", "length", "channels", "ks", "stride", "pad",
+                                                      "nblocks", "hidden", "nout")})
+    rng = np.random.default_rng(12)
+    B = 4
+    p = R.init_params(ospec, seed=3)
+    for k in p:
+        if k[0] in "bc":
+            p[k] = (0.05 * rng.standard_normal(p[k].shape)).astype(np.float32)
+    net = InitNet(spec, params=p, device="cuda")
+    x = R.bf16(rng.standard_normal((spec.cin, B, spec.length)))
+    t = rng.standard_normal((spec.nout, B)).astype(np.float32)
+    net.set_input(torch.as_tensor(x, device="cuda").to(torch.bfloat16))
+    net._buffers(B)["t"].copy_(torch.as_tensor(t, device="cuda"))
+    out = net.forward_buffers(B)
+    loss = net.backward_buffers(B)
+    torch.cuda.synchronize()
+    ro, cache = R.forward(ospec, p, x, emulate=True)
+    rl, rg = R.backward(ospec, p, cache, t.astype(np.float64), emulate=True)
+    e_out = rel_l2(out.cpu().numpy(), ro)
+    print(f"initializer at full spec, batch {B}: output rel L2 {e_out:.2e}")
+    assert e_out < 2e-3
+    assert abs(float(loss) - rl) <= 2e-3 * rl
+    for k, g in rg.items():
+        e = rel_l2(net.g[k].cpu().numpy(), g)
+        assert e < 2e-3, (k, e)
