@@ -742,6 +742,43 @@ def misfit(g_mod, g_obs):
     return r, 0.5 * float(np.sum(r * r))
 
 
+def xcorr_residual(g_mod, g_obs):
+    """computeResidualCrossCorrelation, M:2145-2179, per trace (component c,
+    receiver r) of (3, nt, nrec) gathers: with a = ⟨m, o⟩, |m|, |o| over time,
+
+        res = m·a/(|o|·|m|³) - o/(|m|·|o|)
+
+    (the derivative of 1 - a/(|m||o|) with respect to m), where a zero |m|, |o|
+    or a is replaced by 1 as the reference does.  Also returns the per-trace
+    cost Σ (1 - a/(|m||o|)) with the same guarded norms (raw a), which that
+    residual differentiates."""
+    m = np.asarray(g_mod, np.float64)
+    o = np.asarray(g_obs, np.float64)
+    nm = np.sqrt(np.sum(m * m, axis=-2, keepdims=True))
+    no = np.sqrt(np.sum(o * o, axis=-2, keepdims=True))
+    a = np.sum(m * o, axis=-2, keepdims=True)
+    nm = np.where(nm == 0, 1.0, nm)
+    no = np.where(no == 0, 1.0, no)
+    cost = float(np.sum(1.0 - a / (nm * no)))
+    ag = np.where(a == 0, 1.0, a)
+    res = m * (ag / (no * nm ** 3)) - o / (nm * no)
+    return res, cost
+
+
+def xcorr_cost(g_mod, g_obs):
+    """computeCostFunctionCrossCorrelation, M:2181-2206: per component, one
+    minus the normalised correlation of the whole (nt, nrec) gathers (zero norms
+    replaced by 1), summed over the three components."""
+    m = np.asarray(g_mod, np.float64)
+    o = np.asarray(g_obs, np.float64)
+    total = 0.0
+    for c in range(3):
+        nm = np.sqrt(np.sum(m[c] * m[c])) or 1.0
+        no = np.sqrt(np.sum(o[c] * o[c])) or 1.0
+        total += 1.0 - float(np.sum(m[c] * o[c])) / (nm * no)
+    return total
+
+
 def default_threads():
     try:
         return len(os.sched_getaffinity(0))

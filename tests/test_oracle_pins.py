@@ -26,6 +26,7 @@ from scipy.stats import hmean
 
 from elastic_fm_vp_vs_rho_amd import model as Mo
 from oracle import elastic_fp64 as O
+from tests.problems import tiny
 
 pytestmark = pytest.mark.filterwarnings("error::RuntimeWarning")
 
@@ -270,3 +271,97 @@ def test_cpml_profile_physics(impl, W, N, f, dt, di, vmax):
 def test_cpml_profile_without_layer(impl):
     a, b = IMPLS[impl](10.0, 1e-3, 10.0, 0, 3000.0, 12)
     assert not np.any(a) and not np.any(b)
+
+
+# ---------------------------------------------------------------------------
+# cross-correlation misfit (M:2145-2206)
+# ---------------------------------------------------------------------------
+def test_xcorr_residual_is_the_derivative_of_the_trace_cost():
+    rng = np.random.default_rng(7)
+    m = rng.standard_normal((3, 40, 5))
+    o = rng.standard_normal((3, 40, 5))
+    res, J = O.xcorr_residual(m, o)
+    d = rng.standard_normal(m.shape)
+    h = 1e-6
+    fd = (O.xcorr_residual(m + h * d, o)[1] - O.xcorr_residual(m - h * d, o)[1]) / (2 * h)
+    assert abs(fd - float(np.sum(res * d))) <= 1e-7 * abs(fd)
+    # the cost of one trace is 1 - cos(angle between m and o)
+    c = m[1, :, 2] @ o[1, :, 2] / (np.linalg.norm(m[1, :, 2]) * np.linalg.norm(o[1, :, 2]))
+    one = O.xcorr_residual(m[1:2, :, 2:3], o[1:2, :, 2:3])[1]
+    assert one == pytest.approx(1 - c, rel=1e-12)
+
+
+def test_xcorr_is_blind_to_amplitude():
+    rng = np.random.default_rng(8)
+    o = rng.standard_normal((3, 30, 4))
+    res, J = O.xcorr_residual(3.7 * o, o)
+    assert J == pytest.approx(0.0, abs=1e-12)
+    assert np.abs(res).max() < 1e-12 * np.abs(o).max()
+    assert O.xcorr_cost(2.0 * o, o) == pytest.approx(0.0, abs=1e-12)
+    assert O.xcorr_cost(-o, o) == pytest.approx(6.0)          # 1 - (-1) per component
+
+
+def test_xcorr_guards():
+    """A silent modelled trace: |m| and ⟨m, o⟩ are replaced by 1, so the residual
+    is -o/|o| there (M:2155-2164)."""
+    o = np.random.default_rng(9).standard_normal((3, 20, 2))
+    m = np.zeros_like(o)
+    res, J = O.xcorr_residual(m, o)
+    np.testing.assert_allclose(res, -o / np.linalg.norm(o, axis=1, keepdims=True), rtol=1e-12)
+    assert J == pytest.approx(6.0)
+    assert O.xcorr_cost(m, m) == pytest.approx(3.0)
+
+
+# ---------------------------------------------------------------------------
+# Wavelet.gaussian (M:1274-1281) and the reference adjoint's injection (M:310-322)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_gaussian_wavelet(impl):
+    """A unit-height Gaussian centred at t0 = 1/f (in samples 1/(f·dt)) whose
+    standard deviation is 1/(2πf) seconds: exp(-2π²f²τ²) = exp(-τ²/(2s²))."""
+    fn = {"oracle": O.gaussian, "product": Mo.gaussian}[impl]
+    f, dt, nt = 7.0, 1e-3, 600
+    w = fn(f, dt, nt).astype(np.float64)
+    t = np.arange(nt) * dt
+    t0 = 1.0 / f
+    s = 1.0 / (2 * np.pi * f)
+    np.testing.assert_allclose(w, np.exp(-(t - t0) ** 2 / (2 * s * s)), rtol=1e-6, atol=1e-7)
+    assert abs(int(np.argmax(w)) - round(t0 / dt)) <= 1
+
+
+def test_adjoint_source_is_injected_time_reversed_after_the_sweeps():
+    """backwardFields (M:2056-2082): iteration it runs the sweeps, then adds
+    res[c, nt-1-it] to v_c at each receiver (kernel_add_adjoint_source,
+    M:310-322).  A residual that is non-zero only at sample k therefore leaves
+    the state zero before iteration nt-1-k and exactly that sample at the
+    receiver's (y, z, x) after it; its MT image at the source is the compliance
+    of the adjoint stress, zero while the stress there is zero."""
+    p = tiny(n=(12, 11, 10), nt=9, W=0, seed=31, nrec=3)
+    med, cp = p.medium(), O.Cpml.zeros(p.nx, p.ny, p.nz)
+    rec = np.array([(3, 4, 5), (6, 5, 4), (7, 7, 6)])
+    k, c, r = 6, 1, 2
+    res = np.zeros((3, p.nt, len(rec)))
+    res[c, k, r] = 2.5
+    B = O.Fields(med.shape)
+    for it in range(p.nt - k):                 # iterations 0 .. nt-1-k
+        O.stress_sweep(B, med, cp, p.c)
+        O.velocity_sweep(B, med, cp, p.c)
+        O.add_adjoint_source(B, rec, res, it, p.nt)
+        if it < p.nt - 1 - k:
+            assert all(not np.any(B.f[f]) for f in O.FIELDS), it
+    x, y, z = rec[r]
+    for f in O.FIELDS:
+        want = np.zeros(med.shape)
+        if f == ("vx", "vy", "vz")[c]:
+            want[y, z, x] = 2.5
+        np.testing.assert_array_equal(B.f[f], want, err_msg=f)
+    g = O.image_mt(B, med.lam, med.mu, p.src, 1.0)
+    assert not np.any(g)
+    # linearity of the whole reference MT adjoint in the residual
+    r1 = np.random.default_rng(1).standard_normal((3, p.nt, len(rec)))
+    r2 = np.random.default_rng(2).standard_normal((3, p.nt, len(rec)))
+    w = p.wavelet
+    a = O.ref_backward_mt(med, cp, p.c, p.src, w, rec, p.nt, r1)
+    b = O.ref_backward_mt(med, cp, p.c, p.src, w, rec, p.nt, r2)
+    ab = O.ref_backward_mt(med, cp, p.c, p.src, w, rec, p.nt, 2 * r1 - 3 * r2)
+    np.testing.assert_allclose(ab, 2 * a - 3 * b, rtol=1e-10, atol=1e-12 * np.abs(ab).max())
