@@ -74,6 +74,8 @@ _SIGS = {
     "efm_adjoint": ([P(EfmDims), P(EfmMedium), c_void_p, c_void_p, P(EfmShots), P(P(EfmState)),
                      P(EfmState), P(EfmState), c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                      c_void_p], c_int),
+    "efm_gradient_static": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(P(EfmState)), P(EfmState)]
+                            + [c_void_p] * 6, c_int),
     "efm_gradient_combine": ([P(EfmDims), P(EfmMedium), c_void_p, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p], c_int),
     "efm_inject_residual": ([P(EfmDims), P(EfmShots), P(EfmState), c_void_p, c_int, c_void_p], c_int),

@@ -19,6 +19,8 @@
 // rewrites it, so C is ping-ponged between two buffers.  The derived medium
 // (μ̄, b̄) is zero outside the interior and τ̄, v̄, C are only ever written on
 // the interior, so neighbour terms vanish there without branches.
+#include <mutex>
+#include <string>
 #include <utility>
 
 #include "common.hpp"
@@ -635,30 +637,17 @@ extern "C" int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *m
     return check_launch("derived medium");
 }
 
-extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mbar,
-                           const float *bbar, const EfmShots *sh, const EfmState *const *fwd,
-                           const EfmState *adj, const EfmState *adj2, const float *resid,
-                           float *acc, double *gm6, int n0, int n1, int last, void *stream) {
-    if (!d || !m || !mbar || !bbar || !sh || !fwd || !adj || !adj2 || (!resid && d->nrec > 0) ||
-        !acc || !gm6)
-        return fail(EFM_E_ARG, "efm_adjoint: null argument");
-    if (int rc = validate(d, m, sh)) return rc;
-    if (n0 < 0 || n1 <= n0 || n1 > d->ntmax) return fail(EFM_E_ARG, "efm_adjoint: bad step range");
-    if (int rc = validate_state(d, adj)) return rc;
-    // adj2 carries only the second ψ buffer
-    if ((d->wx_lo + d->wx_hi > 0 && !adj2->psi_x) || (d->wy_lo + d->wy_hi > 0 && !adj2->psi_y) ||
-        (d->wz_lo + d->wz_hi > 0 && !adj2->psi_z))
-        return fail(EFM_E_ARG, "efm_adjoint: adj2 psi slab pointer is null");
-    for (int k = 0; k <= n1 - n0; ++k)
-        if (int rc = validate_state(d, fwd[k])) return rc;
-    const Geo g = make_geo(*d);
-    hipStream_t st = (hipStream_t)stream;
-    AdjArgs A;
-    A.lam = m->lam; A.mu = m->mu; A.mbar = mbar; A.bbar = bbar; A.acc = acc;
-    A.sh = make_shots(*sh);
+namespace efm {
+
+// The reverse sweep of steps n1-1 .. n0 (the body of efm_adjoint, shared with
+// efm_gradient_static).  `copy_back` leaves the live ψ adjoint in adj's
+// buffers when the ping-pong ended in adj2's.
+static int run_adjoint(const EfmDims *d, const Geo &g, AdjArgs A, const EfmState *const *fwd,
+                       const EfmState *adj, const EfmState *adj2, int n0, int n1, int last,
+                       bool copy_back, hipStream_t st) {
     StatePtrs live = make_state(*adj), other = make_state(*adj2);
-    A.resid = resid;
-    A.gm6 = gm6;
+    const float *resid = A.resid;
+    double *gm6 = A.gm6;
     const bool fused = A.sh.recmask != nullptr;
     if (fused && n1 == d->ntmax) {
         // first residual injection of the run; the sweeps do the rest
@@ -692,7 +681,7 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
         if (rc) return rc;
     }
     // leave the live ψ adjoint in adj's buffers (copy back when they were swapped)
-    if (live.px != adj->psi_x) {
+    if (copy_back && live.px != adj->psi_x) {
         const long long fx = (long long)g.S * 6 * g.ny * g.nz * g.Lx;
         const long long fy = (long long)g.S * 6 * g.Ly * g.nz * g.nx;
         const long long fz = (long long)g.S * 6 * g.ny * g.Lz * g.nx;
@@ -704,6 +693,33 @@ extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mb
             return fail(EFM_E_HIP, "efm_adjoint: psi copy");
     }
     return EFM_OK;
+}
+
+}  // namespace efm
+
+extern "C" int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mbar,
+                           const float *bbar, const EfmShots *sh, const EfmState *const *fwd,
+                           const EfmState *adj, const EfmState *adj2, const float *resid,
+                           float *acc, double *gm6, int n0, int n1, int last, void *stream) {
+    if (!d || !m || !mbar || !bbar || !sh || !fwd || !adj || !adj2 || (!resid && d->nrec > 0) ||
+        !acc || !gm6)
+        return fail(EFM_E_ARG, "efm_adjoint: null argument");
+    if (int rc = validate(d, m, sh)) return rc;
+    if (n0 < 0 || n1 <= n0 || n1 > d->ntmax) return fail(EFM_E_ARG, "efm_adjoint: bad step range");
+    if (int rc = validate_state(d, adj)) return rc;
+    // adj2 carries only the second ψ buffer
+    if ((d->wx_lo + d->wx_hi > 0 && !adj2->psi_x) || (d->wy_lo + d->wy_hi > 0 && !adj2->psi_y) ||
+        (d->wz_lo + d->wz_hi > 0 && !adj2->psi_z))
+        return fail(EFM_E_ARG, "efm_adjoint: adj2 psi slab pointer is null");
+    for (int k = 0; k <= n1 - n0; ++k)
+        if (int rc = validate_state(d, fwd[k])) return rc;
+    const Geo g = make_geo(*d);
+    AdjArgs A;
+    A.lam = m->lam; A.mu = m->mu; A.mbar = mbar; A.bbar = bbar; A.acc = acc;
+    A.sh = make_shots(*sh);
+    A.resid = resid;
+    A.gm6 = gm6;
+    return run_adjoint(d, g, A, fwd, adj, adj2, n0, n1, last, true, (hipStream_t)stream);
 }
 
 extern "C" int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const float *mbar,
@@ -746,4 +762,105 @@ extern "C" int efm_model_lame_adjoint(const float *vp, const float *vs, const fl
     hipLaunchKernelGGL(k_lame_adjoint, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
                        vp, vs, rho, glam, gmu, grho_in, gvp, gvs, grho_out, n);
     return check_launch("model lame adjoint");
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free whole gradient (no reference counterpart): the derived medium,
+// the adjoint accumulators and the second ψ-adjoint buffer live in a
+// file-scope device array instead of caller workspaces.
+// ---------------------------------------------------------------------------
+namespace efm {
+
+constexpr long long kScratchFloats = 16LL << 20;   // 64 MiB per device
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_dev[kMaxDevices];   // device address of `scratch`, per device
+
+// The calling thread's current device's copy of `scratch` (looked up once per
+// device, with that device current).
+static int scratch_address(float **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(EFM_E_HIP, "efm_gradient_static: no current device");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(EFM_E_HIP, std::string("efm_gradient_static: hipGetSymbolAddress: ") +
+                                       hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return EFM_OK;
+}
+
+template <typename T>
+__global__ void k_fill_zero(T *p, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = T(0);
+}
+
+template <typename T>
+static int zero_fill(T *p, long long n, hipStream_t st) {
+    if (n <= 0 || !p) return EFM_OK;
+    const long long b = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)(b < 65536 ? b : 65536)), dim3(256), 0, st, p, n);
+    return check_launch("zero fill");
+}
+
+}  // namespace efm
+
+extern "C" int efm_gradient_static(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                                   const EfmState *const *fwd, const EfmState *adj,
+                                   const float *resid, float *glam, float *gmu, float *grho,
+                                   double *gm6, void *stream) {
+    if (!d || !m || !sh || !fwd || !adj || (!resid && d->nrec > 0) || !glam || !gmu || !grho || !gm6)
+        return fail(EFM_E_ARG, "efm_gradient_static: null argument");
+    if (int rc = validate(d, m, sh)) return rc;
+    if (int rc = validate_state(d, adj)) return rc;
+    for (int k = 0; k <= d->ntmax; ++k)
+        if (int rc = validate_state(d, fwd[k])) return rc;
+    const Geo g = make_geo(*d);
+    const long long psi = (long long)g.S * 6 * ((long long)g.pxc + g.pyc + g.pzc);
+    const long long need = (6 + 8LL * g.S) * g.ps + psi;
+    if (need > kScratchFloats)
+        return fail(EFM_E_ARG, "efm_gradient_static: the batch needs " + std::to_string(need * 4) +
+                                   " bytes of scratch, above the library's static " +
+                                   std::to_string(kScratchFloats * 4) + " (use efm_adjoint with "
+                                   "caller workspaces)");
+    float *base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    float *mbar = base, *bbar = base + 3 * g.ps, *acc = base + 6 * g.ps;
+    float *psi2 = acc + 8LL * g.S * g.ps;
+    // what must start at zero: accumulators, the second ψ buffer, the adjoint
+    // state and ∂J/∂M (fill kernels, no memset: callers may capture this call)
+    int rc = zero_fill(acc, 8LL * g.S * g.ps + psi, st);
+    const long long vol = (long long)g.S * g.N;
+    float *fields[9] = {adj->vx, adj->vy, adj->vz, adj->txx, adj->tyy, adj->tzz, adj->txy, adj->txz, adj->tyz};
+    for (int k = 0; k < 9 && !rc; ++k) rc = zero_fill(fields[k], vol, st);
+    if (!rc) rc = zero_fill(adj->psi_x, (long long)g.S * 6 * g.pxc, st);
+    if (!rc) rc = zero_fill(adj->psi_y, (long long)g.S * 6 * g.pyc, st);
+    if (!rc) rc = zero_fill(adj->psi_z, (long long)g.S * 6 * g.pzc, st);
+    if (!rc) rc = zero_fill(gm6, 6LL * g.S, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_derived, dim3(blocks_for(g.N, 256)), dim3(256), 0, st, g, m->mu, m->rho, mbar, bbar);
+    if ((rc = check_launch("derived medium"))) return rc;
+    EfmState second = *adj;
+    second.psi_x = psi2;
+    second.psi_y = psi2 + (long long)g.S * 6 * g.pxc;
+    second.psi_z = second.psi_y + (long long)g.S * 6 * g.pyc;
+    AdjArgs A;
+    A.lam = m->lam; A.mu = m->mu; A.mbar = mbar; A.bbar = bbar; A.acc = acc;
+    A.sh = make_shots(*sh);
+    A.resid = resid;
+    A.gm6 = gm6;
+    if ((rc = run_adjoint(d, g, A, fwd, adj, &second, 0, d->ntmax, 1, false, st))) return rc;
+    hipLaunchKernelGGL(k_combine, dim3(blocks_for(g.N, 256)), dim3(256), 0, st, g, m->mu, mbar, bbar,
+                       acc, glam, gmu, grho);
+    return check_launch("gradient combine");
 }

@@ -182,6 +182,22 @@ int efm_adjoint(const EfmDims *d, const EfmMedium *m, const float *mbar, const f
                 const EfmShots *sh, const EfmState *const *fwd, const EfmState *adj,
                 const EfmState *adj2, const float *resid, float *acc, double *gm6,
                 int n0, int n1, int last, void *stream);
+/* The whole exact gradient of a stored forward history without any device
+ * allocation and without caller workspaces.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  fwd[k] = S^k for k = 0 .. ntmax (every pre-step state);
+ * adj is overwritten (zeroed, then the adjoint state); resid as for
+ * efm_adjoint; glam, gmu, grho and gm6 are written (not accumulated).  The
+ * derived medium, the accumulators and the second ψ-adjoint buffer live in one
+ * static device array of 64 MiB per GPU, held until the process ends:
+ * (6 + 8·S)·(N + EFM_PAD) + 6·S·(ny·nz·Lx + Ly·nz·nx + ny·Lz·nx) floats must
+ * fit in 16 Mi floats (one 96³ shot with 10-cell layers does), else EFM_E_ARG.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (two streams or threads) race and must be serialised by the caller.  Uses
+ * fill kernels, not memsets, so that it can be captured. */
+int efm_gradient_static(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                        const EfmState *const *fwd, const EfmState *adj, const float *resid,
+                        float *glam, float *gmu, float *grho, double *gm6, void *stream);
 /* ∂J/∂λ, ∂J/∂μ, ∂J/∂ρ (ny, nz, nx) summed over shots from acc. */
 int efm_gradient_combine(const EfmDims *d, const EfmMedium *m, const float *mbar,
                          const float *bbar, const float *acc, float *glam, float *gmu,
