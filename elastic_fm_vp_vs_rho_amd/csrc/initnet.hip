@@ -3,15 +3,16 @@
 // reference, whose inversions start from a fixed moment tensor.
 //
 // BF16 operands are uint16 upper halves of IEEE float32 (round to nearest
-// even); products accumulate in FP32 on the matrix cores
-// (v_mfma_f32_16x16x16_bf16).  Activations are (C, B, L).
+// even); products accumulate in FP32 on the matrix cores with gfx950's
+// v_mfma_f32_16x16x32_bf16 (one 32-deep k step per instruction).
+// Activations are (C, B, L).
 #include "common.hpp"
 #include "efm_initnet.h"
 
 namespace efm {
 namespace net {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -29,12 +30,29 @@ inline int blocks_for(long long n, int bs) {
 
 // ---- GEMM ---------------------------------------------------------------------
 // Block tile 64 (m) x 64 (n), k-slab 32, 4 waves; wave w owns the 32 x 32
-// quadrant (w/2, w%2) as 2 x 2 MFMA 16x16 tiles.  Operands are staged through
-// LDS as [row][k] (A) and [col][k] (B) so that each lane reads its 4
-// consecutive k values as one 8-byte word.  MFMA 16x16x16 lane layout: lane l
-// supplies A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16], j < 4, and receives
-// D[4(l/16)+j][l%16].
+// quadrant (w/2, w%2) as 2 x 2 MFMA 16x16 tiles, one 16x16x32 instruction per
+// tile and k-slab.  Operands are staged through LDS as [row][k] (A) and
+// [col][k] (B) so that each lane reads its 8 consecutive k values as one
+// 16-byte word.  MFMA 16x16x32 lane layout: lane l supplies A[l%16][8(l/16)+j]
+// and B[8(l/16)+j][l%16], j < 8, and receives D[4(l/16)+j][l%16], j < 4.
 constexpr int GBM = 64, GBN = 64, GBK = 32, GLD = GBK + 8;
+
+// The 2 x 2 MFMA tiles of one k-slab from the LDS stages.
+__device__ __forceinline__ void mfma_slab(const uint16_t *As, const uint16_t *Bs, int wm, int wn,
+                                          int lane, f32x4 (&acc)[2][2]) {
+    bf16x8 a[2], b[2];
+    const int ko = 8 * (lane >> 4);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm + 16 * i + (lane & 15)) * GLD + ko]);
+        b[i] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn + 16 * i + (lane & 15)) * GLD + ko]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+}
 
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t *__restrict__ A,
                                               long long sam, long long sak,
@@ -76,21 +94,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
             Bs[c * GLD + k] = v;
         }
         __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GBK; kk += 16) {
-            bf16x4 a[2], b[2];
-            const int ko = kk + 4 * (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = *reinterpret_cast<const bf16x4 *>(&As[(wm + 16 * i + (lane & 15)) * GLD + ko]);
-                b[i] = *reinterpret_cast<const bf16x4 *>(&Bs[(wn + 16 * i + (lane & 15)) * GLD + ko]);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+        mfma_slab(As, Bs, wm, wn, lane, acc);
         __syncthreads();
     }
 #pragma unroll
@@ -209,21 +213,7 @@ __global__ __launch_bounds__(256) void k_gemm_vec(int M, int N, int K,
     for (int k0 = 0; k0 < K; k0 += GBK) {
         const bool more = k0 + GBK < K;
         if (more) gemm_fetch<AK, BN>(va, vb, t, m0, n0, k0 + GBK, M, N, K, A, lda, B, ldb);
-#pragma unroll
-        for (int kk = 0; kk < GBK; kk += 16) {
-            bf16x4 a[2], b[2];
-            const int ko = kk + 4 * (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = *reinterpret_cast<const bf16x4 *>(&As[cur][(wm + 16 * i + (lane & 15)) * GLD + ko]);
-                b[i] = *reinterpret_cast<const bf16x4 *>(&Bs[cur][(wn + 16 * i + (lane & 15)) * GLD + ko]);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+        mfma_slab(As[cur], Bs[cur], wm, wn, lane, acc);
         if (more) gemm_stage<AK, BN>(As[cur ^ 1], Bs[cur ^ 1], va, vb, t);
         __syncthreads();
         cur ^= 1;

@@ -217,3 +217,29 @@ def test_training_on_propagator_data_learns(gpu_lib):
         assert 2 <= len(J) <= 4 and np.isfinite(J).all()
         assert all(b < a for a, b in zip(J, J[1:])), J
     assert set(jab["iters_to_tol"]) == {"plain@0.5", "network@0.5"}
+
+
+def test_network_start_needs_fewer_lbfgs_iterations(gpu_lib):
+    """The A/B the initializer exists for, at a reduced size (40³, 2048
+    training shots of 256 steps, 16 receivers): on 8 held-out shots at their
+    catalogue locations, L-BFGS on M from the network's proposal reaches
+    1e-2·J0 in fewer iterations (median) than from the reference's fixed start
+    Mmod = (0.5, 0, 0.5, 0, 0.5, 0) (M:2777), J0 being the plain start's misfit,
+    and is never worse than it on more than two shots."""
+    from elastic_fm_vp_vs_rho_amd import initnet_data as D
+    from elastic_fm_vp_vs_rho_amd import workloads
+    from elastic_fm_vp_vs_rho_amd.initnet import InitNet, NetSpec
+    S, held = 2048, 8
+    wl = workloads.c5(n=40, S=S + held, nt=256, nrec=16)
+    c5 = D.C5(wl, "cuda")
+    data, tg, _ = c5.generate(np.arange(S + held), 256, batch=256)
+    spec = NetSpec(cin=48, length=256, channels=32, ks=9, stride=4, pad=4, nblocks=3, hidden=64)
+    net = InitNet(spec, seed=1, device="cuda")
+    D.train(net, data, tg, np.arange(128, S), np.arange(128), epochs=40, batch=64, lr=2e-3)
+    heldout = np.arange(S, S + held)
+    ab = D.ab_test(c5, net, data[S:], heldout, iters=12, tols=(1e-2,))
+    it = lambda who: np.array([r["iters_to_tol"][f"catalogue/{who}@0.01"] or 13 for r in ab])
+    plain, network = it("plain"), it("network")
+    print(f"iterations to 1e-2 J0, plain {plain.tolist()} network {network.tolist()}")
+    assert np.median(network) < np.median(plain), (plain, network)
+    assert int(np.sum(network > plain)) <= 2, (plain, network)

@@ -124,10 +124,19 @@ def test_c5_network_at_full_spec_against_fp64(gpu_lib):
     torch.cuda.synchronize()
     ro, cache = R.forward(ospec, p, x, emulate=True)
     rl, rg = R.backward(ospec, p, cache, t.astype(np.float64), emulate=True)
+    # The device and the restatement round every activation to BF16 but accumulate in a
+    # different order, so an activation that sits on a rounding boundary lands one BF16 ulp
+    # (2^-8 = 3.9e-3 of its value) apart.  At the full size the restatement itself, with a
+    # relative perturbation of 2e-7 (FP32 accumulation) added before each BF16 rounding, moves
+    # the outputs by 2.7e-3 and the parameter gradients by up to 5.5e-3 (W0) on this batch, the
+    # device's errors almost exactly: tolerance 1e-2 (the small network of test_gpu_initnet.py:
+    # 2e-3).
+    tol = 1e-2
     e_out = rel_l2(out.cpu().numpy(), ro)
     print(f"initializer at full spec, batch {B}: output rel L2 {e_out:.2e}")
-    assert e_out < 2e-3
-    assert abs(float(loss) - rl) <= 2e-3 * rl
+    assert e_out < tol
+    assert abs(float(loss) - rl) <= tol * rl
     for k, g in rg.items():
         e = rel_l2(net.g[k].cpu().numpy(), g)
-        assert e < 2e-3, (k, e)
+        print(f"  d{k}: rel L2 {e:.2e}")
+        assert e < tol, (k, e)
