@@ -279,6 +279,40 @@ def gradient_vs_oracle(n=40, nt=200):
                                f"exact adjoint"}
 
 
+def plan_sample():
+    """Seconds per exact gradient, eager launches against replays from a
+    persistent plan of HIP graphs (after one capturing call), on a batch whose
+    steps are long (96^3 x 64 shots, 256 steps) and on one whose steps are a few
+    microseconds (40^3 x 1 shot, 512 steps); tiles tuned per batch."""
+    import numpy as np
+    import torch
+    from elastic_fm_vp_vs_rho_amd import autotune, workloads
+    out = {}
+    for n, S, nt in ((96, 64, 256), (40, 1, 512)):
+        wl = workloads.c5(n=n, S=S, nt=nt)
+        res = {}
+        for graphs in (False, True):
+            prop = sample_propagator(wl, S, nt)
+            prop.graphs = graphs
+            if not graphs:
+                autotune.tune(prop)
+            obs = torch.zeros_like(prop.forward()[0])
+            prop.gradient(obs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                prop.gradient(obs)
+            torch.cuda.synchronize()
+            res["graphs" if graphs else "eager"] = (time.perf_counter() - t0) / 3
+            res["strategy"] = prop.last_strategy
+            prop.release_states()
+            del prop
+            torch.cuda.empty_cache()
+        res["eager_over_graphs"] = res["eager"] / res["graphs"]
+        out[f"{n}^3 x {S} shot(s) x {nt} steps"] = res
+    return out
+
+
 def c4_gradient_sample(steps):
     """Seconds per exact gradient of the C4 survey at full size (200x200x100, 16
     shots, one per sub-batch as the FWI driver runs it) with every record cut to
@@ -344,6 +378,11 @@ def main():
     ap.add_argument("--no-store", action="store_true", help="skip timing the store strategy beside it")
     ap.add_argument("--no-parity", action="store_true", help="skip the errors against FP64")
     ap.add_argument("--no-c4", action="store_true", help="skip the bounded C4 gradient sample")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="skip timing the same gradient from a persistent plan of HIP graphs")
+    ap.add_argument("--plan-sample", action="store_true",
+                    help="only time gradients eager vs from a persistent graph plan on 96^3 x 64 "
+                         "shots and on 40^3 x 1 shot, and print that")
     ap.add_argument("--c4-steps", type=int, default=300,
                     help="record length of the bounded C4 gradient sample (16 shots, full grid)")
     ap.add_argument("--keep", type=int, default=None,
@@ -365,6 +404,9 @@ def main():
         c3 = workloads.c3()
         print(json.dumps({"workload": c3.name, "batched_vs_plain":
                           batched_vs_plain(c3, c3.S, int(c3.nt.max()), tune=True)}))
+        return
+    if args.plan_sample:
+        print(json.dumps({"plan_vs_eager": plan_sample()}))
         return
     if args.profile_kernels and args.profile_workload != "c2":
         big = workloads.c3(S=4) if args.profile_workload == "c3x4" else workloads.c4(S=1)
@@ -461,6 +503,23 @@ def main():
         line["store"] = {"s_per_gradient": ts, "value": N * nt * wl.S / ts,
                          "peak_hbm_gb": torch.cuda.max_memory_allocated() / 1e9,
                          "note": "every forward state kept in HBM (no re-forward)"}
+    if not args.no_graphs:
+        gprop = Propagator(grid, lam, mu, f(wl.rho), shots, graphs=True)
+        gprop._pool = prop._pool                     # the same forward-state buffers
+        gprop.gradient(obs, K, strategy, args.keep)  # captures the plan's graphs
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ng = max(1, min(args.steps, 3))
+        for _ in range(ng):
+            gprop.gradient(obs, K, strategy, args.keep)
+        torch.cuda.synchronize()
+        tg = (time.perf_counter() - t1) / ng
+        line["graphs"] = {"s_per_gradient": tg, "value": N * nt * wl.S / tg,
+                          "graphs_held": gprop.graph_plan().graphs(),
+                          "note": "same gradient, time loops replayed from a persistent plan of HIP "
+                                  "graphs (bit-identical results)"}
+        gprop.release_states()
+        del gprop
     if not args.no_parity and os.path.exists(FIXTURE_C2):
         ref = np.load(FIXTURE_C2)["gathers"]
         n400 = ref.shape[1]

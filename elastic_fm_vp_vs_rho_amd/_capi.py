@@ -93,6 +93,14 @@ _SIGS = {
                           c_void_p, c_void_p] + [c_int] * 4 + [c_void_p] * 6, c_int),
     "efm_model_rho_nafe_drake": ([c_void_p, c_void_p, c_ll, c_void_p], c_int),
     "efm_volume_axpy": ([c_float, c_void_p, c_void_p, c_ll, c_void_p], c_int),
+    "efm_xcorr_residual": ([c_void_p] * 4 + [c_int] * 3 + [c_void_p], c_int),
+    "efm_plan_create": ([P(EfmDims), P(EfmMedium), P(EfmShots), c_int, P(c_void_p)], c_int),
+    "efm_plan_destroy": ([c_void_p], c_int),
+    "efm_plan_graphs": ([c_void_p], c_int),
+    "efm_plan_forward": ([c_void_p, P(EfmState), c_void_p, c_int, c_int, c_void_p], c_int),
+    "efm_plan_forward_states": ([c_void_p, P(P(EfmState)), c_void_p, c_int, c_int, c_void_p], c_int),
+    "efm_plan_adjoint": ([c_void_p, c_void_p, c_void_p, P(P(EfmState)), P(EfmState), P(EfmState), c_void_p,
+                          c_void_p, c_void_p, c_int, c_int, c_int, c_void_p], c_int),
     # initializer network (include/efm_initnet.h)
     "efm_net_cast_bf16": ([c_void_p, c_void_p, c_ll, c_void_p], c_int),
     "efm_net_gemm": ([c_int, c_int, c_int, c_void_p, c_ll, c_ll, c_void_p, c_ll, c_ll, c_void_p, c_ll,

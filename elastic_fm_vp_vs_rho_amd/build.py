@@ -22,7 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libefm.so")
-SOURCES = ("forward.hip", "adjoint.hip", "reference.hip", "capi.hip", "initnet.hip", "conditioning.hip")
+SOURCES = ("forward.hip", "adjoint.hip", "reference.hip", "capi.hip", "initnet.hip", "conditioning.hip", "plan.hip")
 ARCH = os.environ.get("EFM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

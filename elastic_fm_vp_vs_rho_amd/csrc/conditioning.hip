@@ -23,7 +23,7 @@
 
 namespace efm {
 
-constexpr int kMaxRadius = 32;
+constexpr int kMaxRadius = 64;
 
 struct GaussTaps {
     double w[kMaxRadius + 1];   // w[j] for j = 0 .. r (symmetric kernel)
@@ -86,6 +86,25 @@ __global__ void k_mask_copy(const float *__restrict__ in, float *__restrict__ ou
     out[i] = masked(mg, x, y, z, nx, ny, nz) ? 0.f : in[i];
 }
 
+// Model.compute_Rho (M:1204-1206) in the reference's float32 numpy arithmetic:
+// v = vp/1e3, the five terms each a float32 product with the powers of v
+// (v² as v·v, v³..v⁵ rounded once from exact double products, as a correctly
+// rounded powf gives), summed left to right in float32, times 1e3.
+__device__ __forceinline__ float nafe_drake_f32(float vp) {
+#pragma clang fp contract(off)
+    const float v = vp / 1000.0f;
+    const double vd = (double)v;
+    const float v2 = v * v;
+    const float v3 = (float)(vd * vd * vd);
+    const float v4 = (float)((vd * vd) * (vd * vd));
+    const float v5 = (float)((vd * vd) * (vd * vd) * vd);
+    float r = 1.6612f * v - 0.4712f * v2;
+    r = r + 0.0671f * v3;
+    r = r - 0.0043f * v4;
+    r = r + 0.000106f * v5;
+    return r * 1000.0f;
+}
+
 // lam = f32(lam0 + α_λ·dlam), mu likewise (double, one rounding: the
 // reference's float32 model plus a float64 step); vp, vs from (lam, mu, rho_in) in
 // float32 (compute_Vp_Vs); rho = Nafe-Drake(vp) (compute_Rho) except at
@@ -111,9 +130,7 @@ __global__ void k_model_update(const float *lam0, const float *dlam, double alam
         if (z < top && rho_keep) {
             rho[i] = rho_keep[i];
         } else {
-            const double v = (double)p / 1e3;
-            const double v2 = v * v, v3 = v2 * v, v4 = v3 * v, v5 = v4 * v;
-            rho[i] = (float)((1.6612 * v - 0.4712 * v2 + 0.0671 * v3 - 0.0043 * v4 + 0.000106 * v5) * 1e3);
+            rho[i] = nafe_drake_f32(p);
         }
     }
 }
@@ -122,15 +139,42 @@ __global__ void k_rho_nafe_drake(const float *vp, float *rho, long long n) {
 #pragma clang fp contract(off)
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double v = (double)vp[i] / 1e3;
-    const double v2 = v * v, v3 = v2 * v, v4 = v3 * v, v5 = v4 * v;
-    rho[i] = (float)((1.6612 * v - 0.4712 * v2 + 0.0671 * v3 - 0.0043 * v4 + 0.000106 * v5) * 1e3);
+    rho[i] = nafe_drake_f32(vp[i]);
 }
 
 __global__ void k_axpy(float a, const float *x, float *y, long long n) {
 #pragma clang fp contract(off)
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = y[i] + a * x[i];
+}
+
+// Cross-correlation residual of one trace per thread (computeResidualCrossCorrelation,
+// M:2145-2179): lanes hold consecutive receivers, so each time sample is one
+// coalesced row read; sums in double.  J += the trace's 1 - a/(|m||o|).
+__global__ void k_xcorr(const float *__restrict__ mod, const float *__restrict__ obs,
+                        float *__restrict__ res, double *J, long long ntraces, int ntmax, int nrec) {
+    const long long tr = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr >= ntraces) return;
+    const long long sc = tr / nrec;
+    const int r = (int)(tr - sc * nrec);
+    const long long base = sc * ntmax * nrec + r;
+    double mm = 0.0, oo = 0.0, mo = 0.0;
+    for (int t = 0; t < ntmax; ++t) {
+        const double m = mod[base + (long long)t * nrec], o = obs[base + (long long)t * nrec];
+        mm += m * m;
+        oo += o * o;
+        mo += m * o;
+    }
+    double nm = sqrt(mm), no = sqrt(oo);
+    if (nm == 0.0) nm = 1.0;
+    if (no == 0.0) no = 1.0;
+    const double ag = mo == 0.0 ? 1.0 : mo;
+    const double k1 = ag / (no * nm * nm * nm), k2 = 1.0 / (nm * no);
+    for (int t = 0; t < ntmax; ++t) {
+        const long long i = base + (long long)t * nrec;
+        res[i] = (float)((double)mod[i] * k1 - (double)obs[i] * k2);
+    }
+    atomicAdd(J, 1.0 - mo / (nm * no));
 }
 
 static inline unsigned nblocks(long long n) { return (unsigned)((n + 255) / 256); }
@@ -154,7 +198,7 @@ extern "C" int efm_condition_gradient(const EfmSmooth *p, int nx, int ny, int nz
     hipStream_t st = (hipStream_t)stream;
     // scipy: radius = int(truncate·σ + 0.5); no filtering when σ is (nearly) zero
     const int r = p->sigma > 1e-15f ? (int)((double)p->truncate * (double)p->sigma + 0.5) : 0;
-    if (r > kMaxRadius) return fail(EFM_E_UNSUPPORTED, "efm_condition_gradient: kernel radius above 32");
+    if (r > kMaxRadius) return fail(EFM_E_UNSUPPORTED, "efm_condition_gradient: kernel radius above 64");
     if (p->sigma <= 1e-15f || r == 0) {
         // a radius-0 Gaussian is the identity (its single weight normalises to 1)
         hipLaunchKernelGGL(k_mask_copy, dim3(nblocks(N)), dim3(256), 0, st, in, out, nx, ny, nz, mg);
@@ -206,4 +250,15 @@ extern "C" int efm_volume_axpy(float a, const float *x, float *y, long long n, v
     if (n == 0) return EFM_OK;
     hipLaunchKernelGGL(k_axpy, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, a, x, y, n);
     return check_launch("axpy");
+}
+
+extern "C" int efm_xcorr_residual(const float *mod, const float *obs, float *res, double *J,
+                                  int nshot, int ntmax, int nrec, void *stream) {
+    if (nshot < 0 || ntmax < 0 || nrec < 0) return fail(EFM_E_ARG, "efm_xcorr_residual: bad size");
+    const long long ntr = 3LL * nshot * nrec;
+    if (ntr == 0 || ntmax == 0) return EFM_OK;
+    if (!mod || !obs || !res || !J) return fail(EFM_E_ARG, "efm_xcorr_residual: null argument");
+    hipLaunchKernelGGL(k_xcorr, dim3(nblocks(ntr)), dim3(256), 0, (hipStream_t)stream, mod, obs, res, J,
+                       ntr, ntmax, nrec);
+    return check_launch("cross-correlation residual");
 }

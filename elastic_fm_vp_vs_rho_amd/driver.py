@@ -376,6 +376,31 @@ class Elastic3D:
         self.totalCostFunction = 0.5 * (np.sum(r.Vx ** 2) + np.sum(r.Vy ** 2) + np.sum(r.Vz ** 2))
         self.costFunction.append(float(self.totalCostFunction))
 
+    # -- cross-correlation misfit (M:2145-2206) ---------------------------------------
+    def computeResidualCrossCorrelation(self):
+        """Per-trace normalised cross-correlation residual (M:2145-2179) on the
+        device (efm_xcorr_residual); resGather gets it, as computeResidual's."""
+        mod = torch.as_tensor(self.modGather.stacked()[None], device=self.device)
+        obs = torch.as_tensor(self.obsGather.stacked()[None], device=self.device)
+        r, J = self._propagator().misfit(mod, obs, "xcorr")
+        host = r[0].cpu().numpy()
+        self.resGather.Vx, self.resGather.Vy, self.resGather.Vz = host[0].copy(), host[1].copy(), host[2].copy()
+        self.resGather.hostToDevice(self.device)
+        self._xcorr_trace_cost = float(J)
+
+    def computeCostFunctionCrossCorrelation(self, iter=None):
+        """M:2181-2206: Σ over components of 1 - ⟨mod, obs⟩/(|mod||obs|) over the
+        whole gathers (zero norms replaced by 1), appended to costFunction."""
+        total = 0.0
+        for c in ("Vx", "Vy", "Vz"):
+            m = getattr(self.modGather, c).astype(np.float64)
+            o = getattr(self.obsGather, c).astype(np.float64)
+            nm = float(np.sqrt(np.sum(m * m))) or 1.0
+            no = float(np.sqrt(np.sum(o * o))) or 1.0
+            total += 1.0 - float(np.sum(m * o)) / (nm * no)
+        self.totalCostFunction = total
+        self.costFunction.append(float(total))
+
     # -- reference-style adjoints -------------------------------------------------
     def _mt_image(self, nsteps):
         prop = self._propagator()
@@ -428,12 +453,13 @@ class Elastic3D:
         self.gk.Rho = grho.cpu().numpy()
 
     # -- exact gradient (no reference counterpart) ----------------------------------
-    def gradient_exact(self, obs=None, K=None):
-        """Exact ∂J/∂λ, ∂J/∂μ, ∂J/∂ρ and ∂J/∂M of J = ½‖d - obs‖² for the current
-        earthquake (obs defaults to obsGather).  Returns (J, dict)."""
+    def gradient_exact(self, obs=None, K=None, misfit="l2"):
+        """Exact ∂J/∂λ, ∂J/∂μ, ∂J/∂ρ and ∂J/∂M of J = ½‖d - obs‖² (or of the
+        cross-correlation misfit, misfit='xcorr') for the current earthquake
+        (obs defaults to obsGather).  Returns (J, dict)."""
         prop = self._propagator()
         o = (obs if obs is not None else self.obsGather).hostToDevice(self.device)
-        J, g, glam, gmu, grho, gm6 = prop.gradient(o, K)
+        J, g, glam, gmu, grho, gm6 = prop.gradient(o, K, misfit=misfit)
         return J, dict(lam=glam, mu=gmu, rho=grho, m6=gm6[0].cpu().numpy(), gathers=g)
 
     # -- MT inversion (M:2084-2101, 2217-2346) ------------------------------------
@@ -514,7 +540,7 @@ class Elastic3D:
             self.obsGather.Vx, self.obsGather.Vy, self.obsGather.Vz = (
                 data_obs[i].Vx.copy(), data_obs[i].Vy.copy(), data_obs[i].Vz.copy())
             if gradient == "exact":
-                J, gr = self.gradient_exact(self.obsGather)
+                J, gr = self.gradient_exact(self.obsGather, misfit=getattr(self, "_misfit_kind", "l2"))
                 total += J
                 # the reference's update adds its image (a descent direction)
                 self._accumulate(tl, gr["lam"], -1.0)
@@ -533,9 +559,7 @@ class Elastic3D:
             modData[i] = d
             if self.iteration == 0:
                 self._save("Gathers/modData0", d)          # every shot, one name (M:2414-2417)
-            self.computeResidual()
-            r = self.resGather
-            total += 0.5 * (np.sum(r.Vx ** 2) + np.sum(r.Vy ** 2) + np.sum(r.Vz ** 2))
+            total += self._residual_and_cost()
             self.backwardFields_FowardFieldsRemake()
             gl, gm, gr_ = self._gk_device
             self._accumulate(tl, gl); self._accumulate(tm, gm); self._accumulate(tr, gr_)
@@ -569,18 +593,28 @@ class Elastic3D:
             self.obsGather.Vx, self.obsGather.Vy, self.obsGather.Vz = (
                 data_obs[i].Vx.copy(), data_obs[i].Vy.copy(), data_obs[i].Vz.copy())
             self.forwardFields("mod")
-            self.computeResidual()
-            r = self.resGather
-            total += 0.5 * (np.sum(r.Vx ** 2) + np.sum(r.Vy ** 2) + np.sum(r.Vz ** 2))
+            total += self._residual_and_cost()
         return total
 
+    def _residual_and_cost(self):
+        """resGather and the shot's misfit: ½‖d - obs‖² (M:2128-2142), or the
+        per-trace cross-correlation cost with its residual (M:2145-2179)."""
+        if getattr(self, "_misfit_kind", "l2") == "xcorr":
+            self.computeResidualCrossCorrelation()
+            return self._xcorr_trace_cost
+        self.computeResidual()
+        r = self.resGather
+        return 0.5 * (np.sum(r.Vx ** 2) + np.sum(r.Vy ** 2) + np.sum(r.Vz ** 2))
+
     def fwi_Vp_Vs_Rho(self, data_obs, listaTM, nIter, flagSave=False, tolFWI=1e-5, lbfgsLayers=10,
-                      ite_cte=2, gradient="reference", nTry=10):
+                      ite_cte=2, gradient="reference", nTry=10, misfit="l2"):
         """λ/μ FWI: per-iteration image over all earthquakes, σ=1 smoothing and
         masks, steepest steps for iter < ite_cte, then L-BFGS with α halving
         from 1 (at most nTry trials); ρ follows Vp through Nafe-Drake.
         ``gradient='exact'`` replaces the reference image by the exact
-        (negated) gradient of the same misfit.  Images, conditioning and model
+        (negated) gradient of the same misfit; ``misfit='xcorr'`` uses the
+        cross-correlation residual (M:2145-2179) in place of d - obs (the
+        per-trace cross-correlation cost is then the one minimised).  Images, conditioning and model
         maps stay on the device; the L-BFGS history is lbfgs.py's host arrays,
         as in the reference."""
         self.costFunction = []
@@ -588,6 +622,7 @@ class Elastic3D:
         skL, ykL = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
         skM, ykM = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
         skR, ykR = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+        self._misfit_kind = misfit
         for it in range(nIter):
             self.iteration = it
             total, tl_d, tm_d, tr_d, modData = self._all_shots_gradient(data_obs, listaTM, gradient)
@@ -673,3 +708,64 @@ class Elastic3D:
         np.save(os.path.join(self.results_dir, f"CostFunction/costFunction_{fq}Hz.npy"), self.costFunction)
         for name, arr in (("gRho", tr), ("gMu", tm), ("gLambda", tl)):
             np.save(os.path.join(self.results_dir, f"Gradients/{name}_{fq}Hz.npy"), arr)
+
+
+# ----------------------------------------------------------------------------
+# the reference's __main__ workflow for the Vp/Vs/ρ inversion (mode 1)
+# ----------------------------------------------------------------------------
+def initial_model(mTarget: Model, sigma=10.0, top=10, device=None):
+    """Starting model of the Vp/Vs/ρ inversion (M:3073-3082): the target's Vp,
+    Vs and ρ smoothed with a σ = 10 Gaussian (scipy defaults, here the device
+    kernel), the top `top` cells of Vp and Vs from the target, ρ from the
+    smoothed Vp by Nafe-Drake with the target's top cells, then λ, μ."""
+    dev = _device(device)
+    cn = Model(mTarget.nz, mTarget.ny, mTarget.nx, mTarget.dz, mTarget.dy, mTarget.dx)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+    smooth = lambda a: _C.condition_gradient(f(a), sigma, 4.0).cpu().numpy()
+    cn.Vp, cn.Vs = smooth(mTarget.Vp), smooth(mTarget.Vs)
+    cn.Vp[:, :top, :] = mTarget.Vp[:, :top, :]
+    cn.Vs[:, :top, :] = mTarget.Vs[:, :top, :]
+    cn.Rho = _C.rho_nafe_drake(f(cn.Vp)).cpu().numpy()
+    cn.Rho[:, :top, :] = mTarget.Rho[:, :top, :]
+    cn.compute_Lambda_Mu()
+    return cn
+
+
+def fwi_multifrequency(mTarget: Model, cn: Model, geometry: Geometry, listTM, Mobs, multFq, tprop,
+                       lenPML, nIteFWI, dt=_M.REF_DT, device=None, results_dir=None, **fwi_kw):
+    """Frequency continuation of the reference's mode-1 run (M:3134-3203): for
+    each frequency of multFq, a Ricker wavelet and a CPML at that frequency (the
+    CPML from the target model's Vmax), observed data modelled in mTarget with
+    Mobs for every earthquake (saved as Gathers/obsData_{fq}Hz.npy), then
+    fwi_Vp_Vs_Rho from the current model cn, whose λ, μ and ρ carry over to the
+    next frequency (cn's Vp and Vs are left as they were, as in the reference,
+    M:3188-3194).  Returns {fq: cost history} and leaves cn updated."""
+    history = {}
+    for fq in multFq:
+        cn.MaxMinVp()
+        wavelet = Wavelet(fq, tprop, mTarget, dt=dt)
+        wavelet.ricker()
+        cpml = CPML(lenPML, wavelet, mTarget)
+        obs_obj = Elastic3D(mTarget, geometry, wavelet, cpml, mode=1, device=device, results_dir=results_dir)
+        obs_obj.setStressTensor(Mobs)
+        obs_obj.setModel(mTarget)
+        obsData = []
+        for i in range(geometry.getLenSismos()):
+            obs_obj.setIdxEarthquake(i)
+            obs_obj.forwardFields("obs")
+            d = Gather(wavelet.nSamples, geometry.sismos[i].getLenStations())
+            d.Vx, d.Vy, d.Vz = obs_obj.getGathers()
+            obsData.append(d)
+        if results_dir:
+            os.makedirs(os.path.join(results_dir, "Gathers"), exist_ok=True)
+            arr = np.empty(len(obsData), dtype=object)
+            arr[:] = obsData
+            np.save(os.path.join(results_dir, f"Gathers/obsData_{fq}Hz.npy"), arr, allow_pickle=True)
+        del obs_obj
+        fwi_obj = Elastic3D(cn, geometry, wavelet, cpml, mode=1, device=device, results_dir=results_dir)
+        fwi_obj.setModel(cn)
+        history[fq] = fwi_obj.fwi_Vp_Vs_Rho(obsData, listTM, nIteFWI, **fwi_kw)
+        cn.Rho = fwi_obj.mk.Rho.copy()
+        cn.Lambda = fwi_obj.mk.Lambda.copy()
+        cn.Mu = fwi_obj.mk.Mu.copy()
+    return history

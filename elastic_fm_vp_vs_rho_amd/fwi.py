@@ -146,7 +146,7 @@ def update_mask(grid: Grid, margin=2):
 
 
 def invert(survey: Survey, vp0, vs0, rho0, m60, obs, iters, batch, layers=7,
-           step=(20.0, 12.0, 10.0, 0.05), tries=6, log=None, smooth=0.0):
+           step=(20.0, 12.0, 10.0, 0.05), tries=6, log=None, smooth=0.0, callback=None):
     """Joint L-BFGS on (Vp, Vs, ρ, MT).  The search variable z maps to the model
     as x = x0 + P z with a fixed diagonal P per block, set at the first gradient
     so that the first steepest step (α = 1) changes Vp, Vs, ρ at most by
@@ -199,6 +199,8 @@ def invert(survey: Survey, vp0, vs0, rho0, m60, obs, iters, batch, layers=7,
             J, gvp, gvs, grho, gm6 = gradient(survey, vp, vs, rho, m6, obs, batch)
             gvp, gvs, grho = condition(gvp), condition(gvs), condition(grho)
         hist.append(J)
+        if callback:
+            callback(it, J)
         if log:
             log(f"[fwi] iteration {it}: J = {J:.6e}")
         if it == iters:
@@ -228,6 +230,91 @@ def invert(survey: Survey, vp0, vs0, rho0, m60, obs, iters, batch, layers=7,
     return hist, (vp, vs, rho, m6)
 
 
+def plain_driver(wl, obs, device):
+    """The reference-structured path on a workload: driver.Elastic3D in mode 1
+    (one earthquake at a time, boundary-saving forward, reverse-time
+    reconstruction with the reference's imaging conditions, host misfit and
+    L-BFGS, M:2349-2719), with the workload's sources, receivers, true moment
+    tensors, wavelet and CPML on every face.  Returns (Elastic3D, observed
+    Gathers, listTM)."""
+    from . import driver as D
+    mdl = D.Model(wl.nz, wl.ny, wl.nx, wl.dx, wl.dx, wl.dx)
+    mdl.Vp, mdl.Vs, mdl.Rho = wl.vp.copy(), wl.vs.copy(), wl.rho.copy()
+    mdl.compute_Lambda_Mu()
+    mdl.VmaxVp = wl.vmax
+    wav = D.Wavelet(float(wl.f0[0]), (wl.ntmax + 0.5) * wl.dt, mdl, dt=wl.dt)
+    wav.ricker()
+    cpml = D.CPML(wl.W, wav, mdl, free_top=wl.free_top)
+    geo = D.Geometry()
+    for s in range(wl.S):
+        e = D.Earthquake(*(int(v) for v in wl.src[s]))
+        for r in wl.rec[s]:
+            e.addStation(D.Station(*(int(v) for v in r)))
+        geo.addSismo(e)
+    e3 = D.Elastic3D(mdl, geo, wav, cpml, mode=1, device=device)
+    host = obs.cpu().numpy()
+    gathers = []
+    for s in range(wl.S):
+        g = D.Gather(wl.ntmax, wl.rec.shape[1])
+        g.Vx, g.Vy, g.Vz = host[s, 0].copy(), host[s, 1].copy(), host[s, 2].copy()
+        gathers.append(g)
+    listTM = [D.StressTensor(*(float(v) for v in wl.m6[s])) for s in range(wl.S)]
+    return e3, gathers, listTM
+
+
+def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, log=None):
+    """Wall time of the full FWI on a 256×256×128 survey: the plain single-shot
+    path (plain_driver, `plain_iters` iterations) against this package's
+    batched joint L-BFGS on the exact gradient (invert), both from the smooth
+    start and with the true moment tensors.  Reports the plain path's final
+    misfit, the time invert needs to reach it, and their ratio."""
+    dev = torch.device("cuda:0")
+    wt = workloads.c4(nx=nx, ny=ny, nz=nz, S=S, nt=nt, anomalies=True)
+    w0 = workloads.c4(nx=nx, ny=ny, nz=nz, S=S, nt=nt, anomalies=False)
+    survey = Survey(w0, dev)
+    batch = survey.default_batch()
+    f = lambda a: _tensor(a, dev)
+    lam0, mu0 = lame(f(w0.vp), f(w0.vs), f(w0.rho))
+    autotune.tune(Propagator(survey.grid, lam0, mu0, f(w0.rho), survey.batch(np.arange(batch), w0.m6)))
+    del lam0, mu0
+    obs = observed(survey, f(wt.vp), f(wt.vs), f(wt.rho), wt.m6, batch)
+    J0 = misfit(survey, f(w0.vp), f(w0.vs), f(w0.rho), w0.m6, obs, batch)
+    # plain path
+    e3, gathers, listTM = plain_driver(w0, obs, dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cost_plain = e3.fwi_Vp_Vs_Rho(gathers, listTM, plain_iters, lbfgsLayers=7, ite_cte=2)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    J_plain = e3._misfit_all(gathers, listTM)                  # misfit of its final model
+    if log:
+        log(f"[compare] plain: {plain_iters} iterations in {t_plain:.1f} s, J {J0:.4e} -> {J_plain:.4e}")
+    e3.release_states()
+    del e3
+    torch.cuda.empty_cache()
+    # batched joint inversion, timed per gradient evaluation
+    marks = []
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist, _ = invert(survey, f(w0.vp), f(w0.vs), f(w0.rho), w0.m6.copy(), obs, iters, batch, log=log,
+                     callback=lambda it, J: marks.append((it, J, time.perf_counter() - t1)))
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t1
+    reach = next((m for m in marks if m[1] <= J_plain), None)
+    return {"workload": f"{nx}x{ny}x{nz}, {S} shots x {nt} steps (record length bounded from 2500), "
+                        f"C4-style layered model with two anomalies",
+            "J0": J0,
+            "plain": {"path": "driver.Elastic3D.fwi_Vp_Vs_Rho(gradient='reference'), one earthquake at a "
+                              "time, reference imaging conditions, host L-BFGS",
+                      "iterations": plain_iters, "wall_s": t_plain, "J_history": cost_plain,
+                      "J_final": J_plain},
+            "batched": {"path": "fwi.invert: joint L-BFGS on Vp, Vs, rho and MT, exact gradient",
+                        "iterations": len(hist) - 1, "wall_s": t_batched, "J_history": hist,
+                        "time_to_plain_final_s": reach[2] if reach else None,
+                        "iterations_to_plain_final": reach[0] if reach else None},
+            "speedup_to_equal_misfit": (t_plain / reach[2]) if reach else None}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--iters", type=int, default=20)
@@ -240,9 +327,17 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true", help="keep the installed tiles")
     ap.add_argument("--gradient-only", action="store_true",
                     help="time two batched gradients and stop (no inversion)")
+    ap.add_argument("--compare-plain", action="store_true",
+                    help="time the reference-structured single-shot FWI against the batched inversion "
+                         "to equal misfit (defaults: 256x256x128, 8 shots, 600 steps)")
+    ap.add_argument("--plain-iters", type=int, default=5)
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
+    if args.compare_plain:
+        print(json.dumps(compare_plain(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt,
+                                       plain_iters=args.plain_iters, iters=args.iters, log=log)))
+        return
     wt = workloads.c4(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt, anomalies=True)
     w0 = workloads.c4(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt, anomalies=False)
     survey = Survey(w0, dev)

@@ -214,14 +214,39 @@ def lame_adjoint(vp, vs, rho, glam, gmu, grho):
     return gvp, gvs, gr
 
 
+class Plan:
+    """Owner of an EfmPlan (efm_plan_create/destroy): the HIP graphs of one shot
+    batch's time loops, captured on first use of a set of buffers and replayed
+    afterwards."""
+
+    def __init__(self, prop, chunk):
+        import weakref
+        self.lib = prop.lib
+        self.handle = ctypes.c_void_p()
+        d, m, sh = prop.dims(), prop.medium(), prop.shots.struct()
+        check(self.lib.efm_plan_create(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), int(chunk),
+                                       ctypes.byref(self.handle)), "efm_plan_create")
+        self.chunk = int(chunk)
+        self._fin = weakref.finalize(self, self.lib.efm_plan_destroy, self.handle)
+
+    def graphs(self):
+        return int(self.lib.efm_plan_graphs(self.handle))
+
+
 class Propagator:
     """Forward modelling and exact-adjoint gradients for a shot batch in one medium."""
 
-    def __init__(self, grid: Grid, lam, mu, rho, shots: ShotBatch, pool=None):
+    def __init__(self, grid: Grid, lam, mu, rho, shots: ShotBatch, pool=None, graphs=False):
         """pool: an optional list of wavefields of this grid and shot count,
         shared with other propagators (the forward states of a gradient are
         taken from it and it grows as needed), so that repeated gradients do not
-        allocate and clear them again."""
+        allocate and clear them again.  graphs: run the time loops of records
+        and gradients from a persistent plan of HIP graphs (captured on the
+        first gradient, replayed by later ones; the workspaces the graphs use
+        are then kept by the propagator between calls)."""
+        self.graphs = bool(graphs)
+        self._plan = None
+        self._work = {}
         self.grid, self.shots = grid, shots
         for t in (lam, mu, rho):
             if t.dtype != torch.float32 or not t.is_contiguous() or tuple(t.shape) != grid.shape:
@@ -260,6 +285,10 @@ class Propagator:
         g = self.new_gathers() if gathers is None else gathers
         n = self.shots.ntmax - it0 if nsteps is None else nsteps
         d, m, sh = self.dims(), self.medium(), self.shots.struct()
+        if self.graphs and borders is None and not graph_chunk:
+            check(self.lib.efm_plan_forward(self.graph_plan().handle, ctypes.byref(st.struct()), _ptr(g), int(it0),
+                                            int(n), _stream()), "efm_plan_forward")
+            return g, st
         if graph_chunk and borders is None:
             if getattr(self, "_gstream", None) is None:
                 self._gstream = torch.cuda.Stream(device=self.lam.device)
@@ -277,6 +306,31 @@ class Propagator:
                                    ctypes.byref(st.struct()), _ptr(g), _ptr(borders), int(W),
                                    int(it0), int(n), _stream()), "efm_forward")
         return g, st
+
+    def graph_plan(self):
+        """The propagator's persistent plan (graphs of chunk = the checkpoint interval)."""
+        if self._plan is None:
+            self._plan = Plan(self, self.checkpoint_interval())
+        return self._plan
+
+    def _kept(self, name, make):
+        """A workspace kept between calls when the time loops run from graphs
+        (a captured graph holds its buffers' addresses); else a new one."""
+        if not self.graphs:
+            return make()
+        t = self._work.get(name)
+        if t is None:
+            t = self._work[name] = make()
+        return t
+
+    def _forward_states(self, arr, g, it0, nsteps):
+        d, md, sh = self.dims(), self.medium(), self.shots.struct()
+        if self.graphs:
+            check(self.lib.efm_plan_forward_states(self.graph_plan().handle, arr, _ptr(g), int(it0), int(nsteps),
+                                                   _stream()), "efm_plan_forward_states")
+        else:
+            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(md), ctypes.byref(sh), arr, _ptr(g),
+                                              int(it0), int(nsteps), _stream()), "efm_forward_states")
 
     def border_buffer(self, W):
         d = self.dims()
@@ -336,9 +390,12 @@ class Propagator:
         return pool[:n]
 
     def release_states(self):
-        """Drop this propagator's hold on its states (a shared pool stays with its owner)."""
+        """Drop this propagator's hold on its states, workspaces and graphs (a
+        shared pool stays with its owner)."""
         self._pool = []
         self._arrays = {}
+        self._work = {}
+        self._plan = None
 
     def _array(self, states):
         """ctypes array of EfmState pointers for `states` (cached per list of
@@ -363,14 +420,13 @@ class Propagator:
             if strategy == "hybrid" and m is None:
                 m = m_fit
         nt = self.shots.ntmax
-        g = self.new_gathers()
+        g = self._kept("gathers", self.new_gathers)
+        if self.graphs:
+            g.zero_()
         if strategy == "store":
             states = self._states(nt + 1)
             states[0].zero_()
-            d, md, sh = self.dims(), self.medium(), self.shots.struct()
-            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(md), ctypes.byref(sh),
-                                              self._array(states), _ptr(g), 0, nt, _stream()),
-                  "efm_forward_states")
+            self._forward_states(self._array(states), g, 0, nt)
             return dict(strategy="store", gathers=g, states=states)
         if strategy == "hybrid":
             if m is None:
@@ -384,7 +440,8 @@ class Propagator:
             K = max(1, math.isqrt(max(ncp - 1, 0)) + 1) if K is None else int(K)
             if m < min(K, ncp):
                 raise ValueError("a hybrid tape keeps at least one checkpoint segment of states")
-            st = self.new_wavefield()
+            st = self._kept("record_state", self.new_wavefield)
+            st.zero_()
             snaps = []
             for n0 in range(0, ncp, K):
                 snaps.append(st.clone())
@@ -392,28 +449,36 @@ class Propagator:
             states = self._states(m + 1)
             states[0].copy_(st)
             del st
-            d, md, sh = self.dims(), self.medium(), self.shots.struct()
-            check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(md), ctypes.byref(sh),
-                                              self._array(states[:m + 1]), _ptr(g), ncp, m, _stream()),
-                  "efm_forward_states")
+            self._forward_states(self._array(states[:m + 1]), g, ncp, m)
             return dict(strategy="hybrid", gathers=g, states=states[:m + 1], snaps=snaps, K=K, ncp=ncp)
         K = self.checkpoint_interval() if K is None else int(K)
-        st = self.new_wavefield()
+        st = self._kept("record_state", self.new_wavefield)
+        st.zero_()
         snaps = []
         for n0 in range(0, nt, K):
             snaps.append(st.clone())
             self.forward(st, g, it0=n0, nsteps=min(K, nt - n0))
         return dict(strategy="checkpoint", gathers=g, snaps=snaps, K=K)
 
-    def misfit(self, gathers, obs):
-        """r = d - obs and J = ½‖r‖² on the device (efm_residual)."""
+    def misfit(self, gathers, obs, kind="l2"):
+        """The misfit J and its residual ∂J/∂gathers on the device: ``l2``
+        r = d - obs, J = ½‖r‖² (efm_residual, M:2128-2142); ``xcorr`` the
+        per-trace normalised cross-correlation J = Σ (1 - ⟨d, obs⟩/(|d||obs|))
+        and the reference's residual for it (efm_xcorr_residual, M:2145-2179)."""
         obs = obs.to(device=self.lam.device, dtype=torch.float32).contiguous()
         if obs.shape != gathers.shape:
             raise ValueError(f"observed data must be {tuple(gathers.shape)}")
-        r = torch.empty_like(gathers)
+        r = self._kept("resid", lambda: torch.empty_like(gathers))
         J = torch.zeros(1, dtype=torch.float64, device=self.lam.device)
-        check(self.lib.efm_residual(_ptr(gathers), _ptr(obs), _ptr(r), _ptr(J), gathers.numel(),
-                                    _stream()), "efm_residual")
+        if kind == "l2":
+            check(self.lib.efm_residual(_ptr(gathers), _ptr(obs), _ptr(r), _ptr(J), gathers.numel(),
+                                        _stream()), "efm_residual")
+        elif kind == "xcorr":
+            S, _, nt, nrec = gathers.shape
+            check(self.lib.efm_xcorr_residual(_ptr(gathers), _ptr(obs), _ptr(r), _ptr(J), S, nt, nrec,
+                                              _stream()), "efm_xcorr_residual")
+        else:
+            raise ValueError(f"unknown misfit {kind!r} (l2 or xcorr)")
         return r, J
 
     def backprop(self, tape, resid):
@@ -425,17 +490,34 @@ class Propagator:
         if tuple(resid.shape) != (s.S, 3, s.ntmax, s.nrec):
             raise ValueError(f"residual must be {(s.S, 3, s.ntmax, s.nrec)}")
         d, m, sh = self.dims(), self.medium(), s.struct()
-        mbar = padded_components(3, g.N, dev)
-        bbar = padded_components(3, g.N, dev)
+        if self.graphs:
+            kept = self._kept("resid", lambda: torch.empty_like(resid))
+            if kept.data_ptr() != resid.data_ptr():
+                kept.copy_(resid)
+            resid = kept
+        mbar = self._kept("mbar", lambda: padded_components(3, g.N, dev))
+        bbar = self._kept("bbar", lambda: padded_components(3, g.N, dev))
         check(self.lib.efm_derived_medium(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
                                           _stream()), "efm_derived_medium")
         nt = s.ntmax
-        adj = self.new_wavefield()
-        adj2 = Wavefield(g, s.S, dev, psi_only_of=adj)
-        acc = padded_components(s.S * 8, g.N, dev)
-        gm6 = torch.zeros((s.S, 6), dtype=torch.float64, device=dev)
+        adj = self._kept("adj", self.new_wavefield)
+        adj2 = self._kept("adj2", lambda: Wavefield(g, s.S, dev, psi_only_of=adj))
+        acc = self._kept("acc", lambda: padded_components(s.S * 8, g.N, dev))
+        gm6 = self._kept("gm6", lambda: torch.zeros((s.S, 6), dtype=torch.float64, device=dev))
+        if self.graphs:                          # kept workspaces start each gradient at zero
+            adj.zero_(); adj2.psi_x.zero_(); adj2.psi_y.zero_(); adj2.psi_z.zero_()
+            # the whole buffer, padding included: the interleaved accumulators run across the
+            # component strides of the padded view
+            torch.as_strided(acc, (acc.shape[0] * acc.stride(0),), (1,)).zero_()
+            gm6.zero_()
 
         def sweep(arr, n0, n1):
+            if self.graphs:
+                check(self.lib.efm_plan_adjoint(self.graph_plan().handle, _ptr(mbar), _ptr(bbar), arr,
+                                                ctypes.byref(adj.struct()), ctypes.byref(adj2.struct()),
+                                                _ptr(resid), _ptr(acc), _ptr(gm6), n0, n1, int(n0 == 0),
+                                                _stream()), "efm_plan_adjoint")
+                return
             check(self.lib.efm_adjoint(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
                                        ctypes.byref(sh), arr, ctypes.byref(adj.struct()),
                                        ctypes.byref(adj2.struct()), _ptr(resid), _ptr(acc), _ptr(gm6),
@@ -455,8 +537,7 @@ class Propagator:
                 k = n1 - n0
                 bufs[0].copy_(snaps[seg])
                 arr = self._array(bufs[:k + 1])
-                check(self.lib.efm_forward_states(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), arr,
-                                                  None, n0, k, _stream()), "efm_forward_states")
+                self._forward_states(arr, None, n0, k)
                 sweep(arr, n0, n1)
         glam = torch.empty(g.shape, dtype=torch.float32, device=dev)
         gmu = torch.empty_like(glam)
@@ -464,7 +545,7 @@ class Propagator:
         check(self.lib.efm_gradient_combine(ctypes.byref(d), ctypes.byref(m), _ptr(mbar), _ptr(bbar),
                                             _ptr(acc), _ptr(glam), _ptr(gmu), _ptr(grho), _stream()),
               "efm_gradient_combine")
-        return glam, gmu, grho, gm6
+        return glam, gmu, grho, (gm6.clone() if self.graphs else gm6)
 
     def forward_checkpointed(self, K=None):
         """Forward run keeping the pre-step state every K steps."""
@@ -474,10 +555,12 @@ class Propagator:
     def gradient_from_snapshots(self, snaps, K, resid):
         return self.backprop(dict(strategy="checkpoint", snaps=snaps, K=K), resid)
 
-    def gradient(self, obs, K=None, strategy="auto", m=None):
-        """Misfit J = ½‖d - obs‖² and its exact gradient (the FWI gradient)."""
+    def gradient(self, obs, K=None, strategy="auto", m=None, misfit="l2"):
+        """Misfit J (½‖d - obs‖², or the cross-correlation misfit with
+        misfit='xcorr') and its exact gradient (the FWI gradient)."""
         tape = self.record(K, strategy, m)
-        resid, J = self.misfit(tape["gathers"], obs)
+        resid, J = self.misfit(tape["gathers"], obs, misfit)
         self.last_strategy = tape["strategy"]
         grads = self.backprop(tape, resid)       # queued before J is read back (no host sync)
-        return (float(J), tape["gathers"]) + grads
+        g = tape["gathers"].clone() if self.graphs else tape["gathers"]
+        return (float(J), g) + grads

@@ -229,11 +229,42 @@ int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, const EfmShots
 /* Size in floats of the boundary buffer efm_forward writes for one shot. */
 long long efm_border_floats(const EfmDims *d, int W);
 
+/* ---- persistent plans: the time loops as replayed HIP graphs ----------------
+ * No reference counterpart (the reference issues 9-29 launches per step from
+ * Python, M:1812-2082).  A plan belongs to one shot batch: efm_plan_create
+ * copies d, m and sh (their arrays must outlive the plan), allocates a device
+ * step counter and a capture stream, and keeps graphs keyed by the buffers,
+ * step range and installed tiles of each call; a call whose key it holds
+ * replays that graph on `stream`, otherwise the same launches as the eager
+ * entry point are captured (not run), instantiated, kept and replayed.  No
+ * call synchronises the host; results are bit-identical to the eager entry
+ * points.  Changing a tile drops the plan's graphs.  One call at a time per
+ * plan (calls are serialised internally). */
+typedef struct EfmPlan EfmPlan;
+int efm_plan_create(const EfmDims *d, const EfmMedium *m, const EfmShots *sh, int chunk,
+                    EfmPlan **out);
+int efm_plan_destroy(EfmPlan *p);
+/* Number of graphs the plan holds. */
+int efm_plan_graphs(const EfmPlan *p);
+/* efm_forward without borders: whole chunks of `chunk` steps replay one graph
+ * per (state, gathers) that reads the step index from the plan's counter; the
+ * remaining steps launch eagerly. */
+int efm_plan_forward(EfmPlan *p, const EfmState *state, float *gathers, int it0, int nsteps,
+                     void *stream);
+/* efm_forward_states as one graph per (states, gathers, it0, nsteps). */
+int efm_plan_forward_states(EfmPlan *p, const EfmState *const *states, float *gathers, int it0,
+                            int nsteps, void *stream);
+/* efm_adjoint as one graph per (buffers, n0, n1, last); the ψ copy-back that
+ * efm_adjoint may end with is a copy node of the graph. */
+int efm_plan_adjoint(EfmPlan *p, const float *mbar, const float *bbar, const EfmState *const *fwd,
+                     const EfmState *adj, const EfmState *adj2, const float *resid, float *acc,
+                     double *gm6, int n0, int n1, int last, void *stream);
+
 /* ---- FWI gradient conditioning and model maps (fwi_Vp_Vs_Rho, M:2479-2504,
  * 2577-2583, 2656-2661, 2688-2694; Model M:1200-1206) -------------------- */
 typedef struct EfmSmooth {
     float sigma;       /* Gaussian σ in cells; 0 = no smoothing (masks only) */
-    float truncate;    /* kernel radius = int(truncate·σ + 0.5) cells, at most 32 (scipy's rule) */
+    float truncate;    /* kernel radius = int(truncate·σ + 0.5) cells, at most 64 (scipy's rule) */
     int margin[6];     /* cells zeroed from the x_lo, x_hi, y_lo, y_hi, z_lo, z_hi faces */
 } EfmSmooth;
 /* out = mask(gaussian_filter(in, σ, mode='reflect', truncate)) of one
@@ -255,6 +286,13 @@ int efm_model_update(const float *lam0, const float *dlam, double alpha_lam, con
                      const float *dmu, double alpha_mu, const float *rho_in, const float *rho_keep, int top,
                      int nx, int ny, int nz, float *lam, float *mu, float *vp, float *vs,
                      float *rho, void *stream);
+/* computeResidualCrossCorrelation (M:2145-2179) for gathers (S, 3, ntmax, nrec):
+ * per trace, with a = <mod, obs>, res = mod·a/(|obs|·|mod|³) - obs/(|mod|·|obs|)
+ * (zero |mod|, |obs| or a replaced by 1, as the reference does), sums in
+ * double; J (a device double the caller zeroes) += Σ over traces of
+ * 1 - a/(|mod|·|obs|), the misfit this residual is the derivative of. */
+int efm_xcorr_residual(const float *mod, const float *obs, float *res, double *J,
+                       int nshot, int ntmax, int nrec, void *stream);
 /* Model.compute_Rho (M:1204-1206): rho = Nafe-Drake(vp), n values. */
 int efm_model_rho_nafe_drake(const float *vp, float *rho, long long n, void *stream);
 /* y += a·x over n floats (per-shot gradient sums, as the reference's

@@ -14,6 +14,12 @@ from elastic_fm_vp_vs_rho_amd import model as Mo
 
 pytestmark = pytest.mark.gpu
 
+# The Nafe-Drake polynomial cancels (its terms reach 6x the result) and numpy's
+# float32 powers v**3..v**5 are not correctly rounded (about 80 % are), so one
+# ulp in a power moves ρ by up to ~1.5e-6 relative; the device rounds the powers
+# correctly and otherwise follows the reference's float32 sequence.
+NAFE_DRAKE_RTOL = 4e-6
+
 
 def _dev(a):
     return torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")
@@ -85,7 +91,7 @@ def test_model_update_matches_numpy(gpu_lib):
     np.testing.assert_array_equal(Vp, eVp)
     np.testing.assert_array_equal(Vs, eVs)
     # compute_Rho below the top cells, ρ kept above
-    np.testing.assert_allclose(R[:, top:], Mo.rho_nafe_drake(eVp)[:, top:], rtol=1e-6)
+    np.testing.assert_allclose(R[:, top:], Mo.rho_nafe_drake(eVp)[:, top:], rtol=NAFE_DRAKE_RTOL)
     np.testing.assert_array_equal(R[:, :top], keep[:, :top])
     # a null direction keeps the model; ρ left out on request
     L2, M2, _, _, R2 = C.model_update(_dev(lam), _dev(mu), _dev(rho), with_rho=False)
@@ -97,7 +103,7 @@ def test_rho_nafe_drake_and_axpy(gpu_lib):
     rng = np.random.default_rng(4)
     vp = (1500 + 5000 * rng.random(10007)).astype(np.float32)
     got = C.rho_nafe_drake(_dev(vp)).cpu().numpy()
-    np.testing.assert_allclose(got, Mo.rho_nafe_drake(vp), rtol=1e-6)
+    np.testing.assert_allclose(got, Mo.rho_nafe_drake(vp), rtol=NAFE_DRAKE_RTOL)
     # published point of the polynomial (Brocher 2005, with the reference's 0.4712)
     assert abs(float(C.rho_nafe_drake(_dev(np.array([3000.0])))[0]) - 2232.0) < 1.0
     x = rng.standard_normal(5000).astype(np.float32)
