@@ -279,38 +279,33 @@ class Propagator:
     def forward(self, state=None, gathers=None, it0=0, nsteps=None, borders=None, W=0,
                 graph_chunk=None):
         """Advance ``state`` (zeroed if None) through steps it0 .. it0+nsteps-1.
-        ``graph_chunk`` = K replays one captured hipGraph of K steps instead of
-        launching every kernel (no border save in that mode)."""
+        ``graph_chunk`` = K replays one captured hipGraph of K steps (from a
+        persistent plan of this propagator, so later calls on the same buffers
+        capture nothing) instead of launching every kernel (no border save in
+        that mode); so does every forward of a propagator made with graphs=True."""
         st = self.new_wavefield() if state is None else state
         g = self.new_gathers() if gathers is None else gathers
         n = self.shots.ntmax - it0 if nsteps is None else nsteps
         d, m, sh = self.dims(), self.medium(), self.shots.struct()
-        if self.graphs and borders is None and not graph_chunk:
-            check(self.lib.efm_plan_forward(self.graph_plan().handle, ctypes.byref(st.struct()), _ptr(g), int(it0),
+        if borders is None and (graph_chunk or self.graphs):
+            plan = self.graph_plan(int(graph_chunk) if graph_chunk else None)
+            check(self.lib.efm_plan_forward(plan.handle, ctypes.byref(st.struct()), _ptr(g), int(it0),
                                             int(n), _stream()), "efm_plan_forward")
-            return g, st
-        if graph_chunk and borders is None:
-            if getattr(self, "_gstream", None) is None:
-                self._gstream = torch.cuda.Stream(device=self.lam.device)
-                self._counter = torch.zeros(1, dtype=torch.int32, device=self.lam.device)
-            cur = torch.cuda.current_stream(self.lam.device)
-            self._gstream.wait_stream(cur)
-            with torch.cuda.stream(self._gstream):
-                check(self.lib.efm_forward_graph(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
-                                                 ctypes.byref(st.struct()), _ptr(g), int(it0), int(n),
-                                                 int(graph_chunk), _ptr(self._counter), _stream()),
-                      "efm_forward_graph")
-            cur.wait_stream(self._gstream)
             return g, st
         check(self.lib.efm_forward(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
                                    ctypes.byref(st.struct()), _ptr(g), _ptr(borders), int(W),
                                    int(it0), int(n), _stream()), "efm_forward")
         return g, st
 
-    def graph_plan(self):
-        """The propagator's persistent plan (graphs of chunk = the checkpoint interval)."""
-        if self._plan is None:
-            self._plan = Plan(self, self.checkpoint_interval())
+    def graph_plan(self, chunk=None):
+        """The propagator's persistent plan whose forward graphs cover `chunk`
+        steps (default: the checkpoint interval, the records' segment length)."""
+        chunk = self.checkpoint_interval() if chunk is None else int(chunk)
+        if self._plan is None or self._plan.chunk != chunk:
+            plans = self.__dict__.setdefault("_plans", {})
+            if chunk not in plans:
+                plans[chunk] = Plan(self, chunk)
+            self._plan = plans[chunk]
         return self._plan
 
     def _kept(self, name, make):
@@ -396,6 +391,7 @@ class Propagator:
         self._arrays = {}
         self._work = {}
         self._plan = None
+        self._plans = {}
 
     def _array(self, states):
         """ctypes array of EfmState pointers for `states` (cached per list of

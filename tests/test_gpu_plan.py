@@ -53,3 +53,26 @@ def test_plan_forward_matches_eager(gpu_lib):
     torch.cuda.synchronize()
     assert torch.equal(g1, g2) and torch.equal(g1, g3)
     assert torch.equal(s1.fields, s2.fields)
+
+
+def test_efm_forward_graph_entry_point_matches_eager(gpu_lib):
+    """The one-shot graph entry point (captures, replays and releases a graph per
+    call, on a non-default stream) still gives the eager forward bit for bit."""
+    import ctypes
+    from elastic_fm_vp_vs_rho_amd import _capi
+    from elastic_fm_vp_vs_rho_amd.propagator import _ptr, _stream
+    p = tiny(n=(22, 20, 18), nt=70, W=3, seed=53, nrec=4)
+    prop = _two_shot_prop(p, graphs=False)
+    g_ref, s_ref = prop.forward()
+    st, g = prop.new_wavefield(), prop.new_gathers()
+    counter = torch.zeros(1, dtype=torch.int32, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    d, m, sh = prop.dims(), prop.medium(), prop.shots.struct()
+    with torch.cuda.stream(side):
+        _capi.check(gpu_lib.efm_forward_graph(ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh),
+                                              ctypes.byref(st.struct()), _ptr(g), 0, p.nt, 16, _ptr(counter),
+                                              _stream()), "efm_forward_graph")
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    assert torch.equal(g, g_ref) and torch.equal(st.fields, s_ref.fields)
