@@ -9,7 +9,7 @@
 //
 //   reverse step n     R1 v̄[rec] += r[n]
 //                      R2 u = c·b̄·v̄;  h̄ = u + a(C+u);  C ← b(C+u);  τ̄ += D_τᵀ h̄
-//                         ρ-accumulators  += v̄·F
+//                         ρ-accumulators  += v̄·(v^{n+1} - v^n)   (= v̄·c·b̄·F)
 //                      R3 M̄ += w[n]·τ̄(src)
 //                      R4 ē = c·Sᵀτ̄;  ḡ = ē + a(C+ē);  C ← b(C+ē);  v̄ += D_vᵀ ḡ
 //                         λ/μ accumulators += τ̄·(∂S/∂λ,μ)e
@@ -89,21 +89,20 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
     float *__restrict__ tb3 = A.a.t[3] + so;
     float *__restrict__ tb4 = A.a.t[4] + so;
     float *__restrict__ tb5 = A.a.t[5] + so;
-    const float *__restrict__ T0 = A.fn1.t[0] + so;
-    const float *__restrict__ T1 = A.fn1.t[1] + so;
-    const float *__restrict__ T2 = A.fn1.t[2] + so;
-    const float *__restrict__ T3 = A.fn1.t[3] + so;
-    const float *__restrict__ T4 = A.fn1.t[4] + so;
-    const float *__restrict__ T5 = A.fn1.t[5] + so;
+    // forward velocities after and before step n: their difference is c·b̄·F, the
+    // velocity update the ρ accumulators need (b̄ and c are applied by k_combine)
+    const float *__restrict__ Wx1 = A.fn1.v[0] + so;
+    const float *__restrict__ Wy1 = A.fn1.v[1] + so;
+    const float *__restrict__ Wz1 = A.fn1.v[2] + so;
+    const float *__restrict__ Wx0 = A.fn.v[0] + so;
+    const float *__restrict__ Wy0 = A.fn.v[1] + so;
+    const float *__restrict__ Wz0 = A.fn.v[2] + so;
     const float *__restrict__ Cx = A.a.px + px_shot(g, s);
     const float *__restrict__ Cy = A.a.py + py_shot(g, s);
     const float *__restrict__ Cz = A.a.pz + pz_shot(g, s);
     float *__restrict__ Cxo = A.a2.px + px_shot(g, s);
     float *__restrict__ Cyo = A.a2.py + py_shot(g, s);
     float *__restrict__ Czo = A.a2.pz + pz_shot(g, s);
-    const float *__restrict__ Fx = A.fn.px + px_shot(g, s);
-    const float *__restrict__ Fy = A.fn.py + py_shot(g, s);
-    const float *__restrict__ Fz = A.fn.pz + pz_shot(g, s);
     // ρ sums: 3 consecutive floats per cell at component offset 4 of the shot
     float *__restrict__ accr = A.acc + ((long long)s * 8 + 4) * g.ps;
 
@@ -187,39 +186,15 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
         tb4[t] += (hb_xzz_0 - hb_xzz_p) + (hb_xzx_0 - hb_xzx_p);
         tb5[t] += (hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p);
 
-        // ρ accumulators: forward F at p from τ^{n+1} (= τ'' of step n) and ψτ^n
-        const float h_xx = LDF(&T0[t + 1]) - LDF(&T0[t]);
-        const float h_yy = LDF(&T1[t + sxz]) - LDF(&T1[t]);
-        const float h_zz = LDF(&T2[t + nx]) - LDF(&T2[t]);
-        const float t3 = LDF(&T3[t]), t4 = LDF(&T4[t]), t5 = LDF(&T5[t]);
-        const float h_xyy = t3 - LDF(&T3[t - sxz]), h_xyx = t3 - LDF(&T3[t - 1]);
-        const float h_xzz = t4 - LDF(&T4[t - nx]), h_xzx = t4 - LDF(&T4[t - 1]);
-        const float h_yzz = t5 - LDF(&T5[t - nx]), h_yzy = t5 - LDF(&T5[t - sxz]);
-        float f_xx = h_xx, f_yy = h_yy, f_zz = h_zz, f_xyy = h_xyy, f_xyx = h_xyx;
-        float f_xzz = h_xzz, f_xzx = h_xzx, f_yzz = h_yzz, f_yzy = h_yzy;
-        if (sx0 >= 0) {
-            const int pi = px_off(g, y, z, sx0) + 3 * csx;
-            f_xx += bx0 * LDF(&Fx[pi]) + ax0 * h_xx;
-            f_xyx += bx0 * LDF(&Fx[pi + csx]) + ax0 * h_xyx;
-            f_xzx += bx0 * LDF(&Fx[pi + 2 * csx]) + ax0 * h_xzx;
-        }
-        if (sy0 >= 0) {
-            const int pi = py_off(g, sy0, z, x) + 3 * csy;
-            f_xyy += by0 * LDF(&Fy[pi]) + ay0 * h_xyy;
-            f_yy += by0 * LDF(&Fy[pi + csy]) + ay0 * h_yy;
-            f_yzy += by0 * LDF(&Fy[pi + 2 * csy]) + ay0 * h_yzy;
-        }
-        if (sz0 >= 0) {
-            const int pi = pz_off(g, y, sz0, x) + 3 * csz;
-            f_xzz += bz0 * LDF(&Fz[pi]) + az0 * h_xzz;
-            f_yzz += bz0 * LDF(&Fz[pi + csz]) + az0 * h_yzz;
-            f_zz += bz0 * LDF(&Fz[pi + 2 * csz]) + az0 * h_zz;
-        }
+        // ρ accumulators: v̄ times this step's forward velocity change
+        const float dwx = LDF(&Wx1[t]) - LDF(&Wx0[t]);
+        const float dwy = LDF(&Wy1[t]) - LDF(&Wy0[t]);
+        const float dwz = LDF(&Wz1[t]) - LDF(&Wz0[t]);
         float *pr = accr + 3 * (long long)t;
         const float r0 = pr[0], r1 = pr[1], r2 = pr[2];
-        pr[0] = r0 + vx0 * (f_xx + f_xyy + f_xzz);
-        pr[1] = r1 + vy0 * (f_xyx + f_yy + f_yzz);
-        pr[2] = r2 + vz0 * (f_xzx + f_yzy + f_zz);
+        pr[0] = r0 + vx0 * dwx;
+        pr[1] = r1 + vy0 * dwy;
+        pr[2] = r2 + vz0 * dwz;
     }
 }
 
@@ -544,17 +519,18 @@ __global__ void k_combine(Geo g, const float *mu, const float *mbar, const float
         shear += sh4(2, mxy, 1, x > 0, sxz, y > 0);
         shear += sh4(3, mxz, 1, x > 0, nx, z > 0);
         shear += sh4(4, myz, sxz, y > 0, nx, z > 0);
-        float r = AC(5, i) * bbx[i] * bbx[i] + AC(6, i) * bby[i] * bby[i]
-                + AC(7, i) * bbz[i] * bbz[i];
-        if (x > 0) r += AC(5, i - 1) * bbx[i - 1] * bbx[i - 1];
-        if (y > 0) r += AC(6, i - sxz) * bby[i - sxz] * bby[i - sxz];
-        if (z > 0) r += AC(7, i - nx) * bbz[i - nx] * bbz[i - nx];
+        // ρ sums hold Σ v̄·Δv = c·b̄·Σ v̄·F; ∂J/∂ρ = -(c/2)·b̄²·Σ v̄·F = -(1/2)·b̄·Σ v̄·Δv
+        // at both cells each averaged buoyancy couples
+        float r = AC(5, i) * bbx[i] + AC(6, i) * bby[i] + AC(7, i) * bbz[i];
+        if (x > 0) r += AC(5, i - 1) * bbx[i - 1];
+        if (y > 0) r += AC(6, i - sxz) * bby[i - sxz];
+        if (z > 0) r += AC(7, i - nx) * bbz[i - nx];
         sr += r;
     }
     const float m = mu[i];
     glam[i] = c * sl;
     gmu[i] = c * (sm + (m != 0.f ? 0.25f * shear / (m * m) : 0.f));
-    grho[i] = -0.5f * c * sr;
+    grho[i] = -0.5f * sr;
 }
 
 // r = mod - obs and J += ½Σr² (block partial sums in double, one atomic per block)
