@@ -86,6 +86,10 @@ _SIGS = {
     "efm_backward_remake_ref": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), c_void_p,
                                  c_void_p, c_int, P(EfmState), c_void_p, c_float, c_float, c_float,
                                  c_float, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "efm_backward_remake_ref_steps": ([P(EfmDims), P(EfmMedium), P(EfmShots), P(EfmState), c_void_p,
+                                       c_void_p, c_int, P(EfmState), c_void_p, c_float, c_float, c_float,
+                                       c_float, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p],
+                                      c_int),
     "efm_border_floats": ([P(EfmDims), c_int], c_ll),
     "efm_condition_gradient": ([P(EfmSmooth), c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                 c_void_p], c_int),

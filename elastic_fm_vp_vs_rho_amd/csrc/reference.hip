@@ -11,6 +11,7 @@
 // M:191-294; kernel_Gradients_Mij, M:337-372).  It keeps an extra copy of v
 // (kernel_DTD_copy, M:501-511); here the reverse velocity sweep writes
 // out of place and the two v buffers swap roles, so no copy is made.
+#include <string>
 #include <utility>
 
 #include "common.hpp"
@@ -197,18 +198,26 @@ extern "C" long long efm_border_floats(const EfmDims *d, int W) {
     return border_floats(make_geo(*d), W);
 }
 
-extern "C" int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
-                                       const EfmState *fwd, float *vpres, const float *borders,
-                                       int W, const EfmState *adj, const float *res, float dt,
-                                       float dx, float dy, float dz, float *glam, float *gmu,
-                                       float *grho, void *stream) {
+// Iterations it0 .. it0+nsteps-1 of the reverse loop.  The present velocity
+// alternates between the forward state's v and vpres (swapped every
+// iteration), so iteration `it` finds it in fwd.v when it is even and in vpres
+// when it is odd: a run split into ranges gives the same launches in the same
+// order as one run over all nt-1 iterations.
+static int backward_remake_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                               const EfmState *fwd, float *vpres, const float *borders, int W,
+                               const EfmState *adj, const float *res, float dt, float dx, float dy,
+                               float dz, float *glam, float *gmu, float *grho, int it0, int nsteps,
+                               void *stream, const char *who) {
     int rc = check_common(d, m, sh);
     if (rc) return rc;
     if (!fwd || !vpres || !borders || !adj || (!res && d->nrec > 0) || !glam || !gmu || !grho)
-        return fail(EFM_E_ARG, "efm_backward_remake_ref: null argument");
+        return fail(EFM_E_ARG, std::string(who) + ": null argument");
     if ((rc = validate_state(d, fwd)) || (rc = validate_state(d, adj))) return rc;
     if (W + H + 1 > d->nx / 2 || W + H + 1 > d->ny / 2 || W + H + 1 > d->nz / 2)
-        return fail(EFM_E_ARG, "efm_backward_remake_ref: CPML too wide for the grid");
+        return fail(EFM_E_ARG, std::string(who) + ": CPML too wide for the grid");
+    const int nt = d->ntmax;
+    if (it0 < 0 || nsteps < 0 || it0 + nsteps > nt - 1)
+        return fail(EFM_E_ARG, std::string(who) + ": iterations outside [0, nt-1)");
     Geo ga = make_geo(*d);
     if (ga.c < 0) ga.c = -ga.c;
     Geo gf = ga;
@@ -220,8 +229,10 @@ extern "C" int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, con
     StatePtrs nxt = cur;                         // nxt.v = the other buffer
     const long long vol = (long long)ga.S * ga.N;
     nxt.v[0] = vpres; nxt.v[1] = vpres + vol; nxt.v[2] = vpres + 2 * vol;
-    const int nt = d->ntmax;
-    for (int it = 0; it < nt - 1; ++it) {
+    if (it0 & 1) {
+        std::swap(cur.v[0], nxt.v[0]); std::swap(cur.v[1], nxt.v[1]); std::swap(cur.v[2], nxt.v[2]);
+    }
+    for (int it = it0; it < it0 + nsteps; ++it) {
         // adjoint step with the forward operator
         if ((rc = launch_stress_src(ga, *m, shp, a, a, it, 0, st))) return rc;
         if ((rc = launch_velocity(ga, *m, shp, a, a, it, st))) return rc;
@@ -243,4 +254,23 @@ extern "C" int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, con
         std::swap(cur.v[0], nxt.v[0]); std::swap(cur.v[1], nxt.v[1]); std::swap(cur.v[2], nxt.v[2]);
     }
     return EFM_OK;
+}
+
+extern "C" int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                                       const EfmState *fwd, float *vpres, const float *borders,
+                                       int W, const EfmState *adj, const float *res, float dt,
+                                       float dx, float dy, float dz, float *glam, float *gmu,
+                                       float *grho, void *stream) {
+    if (!d) return fail(EFM_E_ARG, "efm_backward_remake_ref: null dims");
+    return backward_remake_ref(d, m, sh, fwd, vpres, borders, W, adj, res, dt, dx, dy, dz, glam, gmu,
+                               grho, 0, d->ntmax - 1, stream, "efm_backward_remake_ref");
+}
+
+extern "C" int efm_backward_remake_ref_steps(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                                             const EfmState *fwd, float *vpres, const float *borders,
+                                             int W, const EfmState *adj, const float *res, float dt,
+                                             float dx, float dy, float dz, float *glam, float *gmu,
+                                             float *grho, int it0, int nsteps, void *stream) {
+    return backward_remake_ref(d, m, sh, fwd, vpres, borders, W, adj, res, dt, dx, dy, dz, glam, gmu,
+                               grho, it0, nsteps, stream, "efm_backward_remake_ref_steps");
 }

@@ -299,6 +299,10 @@ class Elastic3D:
         self._borders = None
         self._shots = None
         self._state_pool = []     # forward states of exact gradients, reused across calls
+        # True: drive the reference-style time loops from Python one step at a time, with the
+        # reference's per-step host transfers (its structure, M:1823, 1939-2054), so that the
+        # plain path can be timed as the reference runs it; the results equal the C loops'
+        self.host_loop = False
         self.mk.hostToDevice(self.device)
 
     # -- configuration ---------------------------------------------------------
@@ -352,7 +356,10 @@ class Elastic3D:
         borders = None
         if strname == "mod" and self.mode == 1:
             borders = prop.border_buffer(self.lenPML)
-        g, st = prop.forward(borders=borders, W=self.lenPML)
+        if self.host_loop:
+            g, st = self._forward_per_step(prop, borders)
+        else:
+            g, st = prop.forward(borders=borders, W=self.lenPML)
         self.forwFields, self._borders = st, borders
         host = g[0].cpu().numpy()
         self.gather.Vx, self.gather.Vy, self.gather.Vz = host[0].copy(), host[1].copy(), host[2].copy()
@@ -360,6 +367,17 @@ class Elastic3D:
             self.modGather.Vx, self.modGather.Vy, self.modGather.Vz = self.getGathers()
         if strname == "obs":
             self.obsGather.Vx, self.obsGather.Vy, self.obsGather.Vz = self.getGathers()
+
+    def _forward_per_step(self, prop, borders):
+        """The forward loop as the reference drives it (M:1823-1848): one host
+        call per time step (stress, source, velocity, stations, border save)."""
+        L = _capi.lib()
+        st, g = prop.new_wavefield(), prop.new_gathers()
+        d, m, sh, ss = prop.dims(), prop.medium(), self._shots.struct(), st.struct()
+        by, gp, bp, W, hs = ctypes.byref, _ptr(g), _ptr(borders), int(self.lenPML), _stream()
+        for it in range(self.nt):
+            _capi.check(L.efm_forward(by(d), by(m), by(sh), by(ss), gp, bp, W, it, 1, hs), "efm_forward")
+        return g, st
 
     def getGathers(self):
         return self.gather.Vx.copy(), self.gather.Vy.copy(), self.gather.Vz.copy()
@@ -441,11 +459,27 @@ class Elastic3D:
         gmu, grho = torch.zeros_like(glam), torch.zeros_like(glam)
         res = self.resGather.hostToDevice(self.device)
         d, m, sh = prop.dims(), prop.medium(), self._shots.struct()
-        _capi.check(L.efm_backward_remake_ref(
-            ctypes.byref(d), ctypes.byref(m), ctypes.byref(sh), ctypes.byref(self.forwFields.struct()),
-            _ptr(vpres), _ptr(self._borders), int(self.lenPML), ctypes.byref(adj.struct()), _ptr(res),
-            float(self.ondicula.dt), float(self.dx), float(self.dy), float(self.dz), _ptr(glam), _ptr(gmu),
-            _ptr(grho), _stream()), "efm_backward_remake_ref")
+        by = ctypes.byref
+        args = (by(d), by(m), by(sh), by(self.forwFields.struct()), _ptr(vpres), _ptr(self._borders),
+                int(self.lenPML), by(adj.struct()), _ptr(res), float(self.ondicula.dt), float(self.dx),
+                float(self.dy), float(self.dz), _ptr(glam), _ptr(gmu), _ptr(grho))
+        if self.host_loop:
+            # the reference's loop: one host call per iteration, each iteration's three images
+            # copied to the host and summed there in float32 (M:1939-2054)
+            hl, hm, hr = (np.zeros(self.grid.shape, np.float32) for _ in range(3))
+            for it in range(self.nt - 1):
+                glam.zero_(); gmu.zero_(); grho.zero_()
+                _capi.check(L.efm_backward_remake_ref_steps(*args, it, 1, _stream()),
+                            "efm_backward_remake_ref_steps")
+                hl += glam.cpu().numpy()
+                hm += gmu.cpu().numpy()
+                hr += grho.cpu().numpy()
+            self.forwFields = None
+            self.gk.Lambda, self.gk.Mu, self.gk.Rho = hl, hm, hr
+            f = lambda a: torch.as_tensor(a, device=self.device)
+            self._gk_device = (f(hl), f(hm), f(hr))
+            return
+        _capi.check(L.efm_backward_remake_ref(*args, _stream()), "efm_backward_remake_ref")
         self.forwFields = None          # consumed by the reverse-time reconstruction
         self._gk_device = (glam, gmu, grho)
         self.gk.Lambda = glam.cpu().numpy()

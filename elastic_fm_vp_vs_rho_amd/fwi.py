@@ -262,7 +262,8 @@ def plain_driver(wl, obs, device):
     return e3, gathers, listTM
 
 
-def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, log=None):
+def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, log=None,
+                  host_loop=True):
     """FWI on a 256×256×128 survey: the plain single-shot path (plain_driver,
     one earthquake at a time, host-driven, the reference's iteration rules),
     with the reference's imaging conditions and with the exact gradient, against
@@ -270,7 +271,10 @@ def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, 
     the smooth start with the true moment tensors.  Reports s per FWI gradient
     of each path, the misfit histories and wall times, and the time the batched
     inversion needs to reach each plain run's final misfit (when that run
-    lowered the misfit at all)."""
+    lowered the misfit at all).  host_loop: also time one gradient of the plain
+    path driven the way the reference drives it (one host call per time step,
+    each reverse iteration's three images copied to the host and summed there,
+    M:1823, 1939-2054)."""
     dev = torch.device("cuda:0")
     wt = workloads.c4(nx=nx, ny=ny, nz=nz, S=S, nt=nt, anomalies=True)
     w0 = workloads.c4(nx=nx, ny=ny, nz=nz, S=S, nt=nt, anomalies=False)
@@ -311,6 +315,23 @@ def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, 
         e3.release_states()
         del e3
         torch.cuda.empty_cache()
+    if host_loop:
+        e3, gathers, listTM = plain_driver(w0, obs, dev)
+        e3._misfit_kind = "l2"
+        e3.iteration = 1
+        e3.host_loop = True
+        _, t_host = timed(lambda: e3._all_shots_gradient(gathers, listTM, "reference"))
+        out["plain_host_driven"] = {
+            "path": "driver.Elastic3D with host_loop=True, gradient='reference': one host call per time "
+                    "step, each reverse iteration's three images copied to the host and summed there "
+                    "(the reference's loop structure); one gradient timed, its iterates are those of the "
+                    "'reference' plain run",
+            "s_per_gradient": t_host}
+        if log:
+            log(f"[compare] plain (reference, host-driven): {t_host:.2f} s per gradient")
+        e3.release_states()
+        del e3
+        torch.cuda.empty_cache()
     args = (survey, f(w0.vp), f(w0.vs), f(w0.rho), w0.m6.copy(), obs, batch)
     gradient(*args)                                              # warm-up
     _, t_bgrad = timed(lambda: gradient(*args))
@@ -337,6 +358,9 @@ def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, 
         pl["batched_time_to_this_misfit_s"] = reach[2] if reach else None
         pl["speedup_to_equal_misfit"] = pl["wall_s"] / reach[2] if reach else None
         pl["plain_over_batched_s_per_gradient"] = pl["s_per_gradient"] / t_bgrad
+    if "plain_host_driven" in out:
+        out["plain_host_driven"]["plain_over_batched_s_per_gradient"] = (
+            out["plain_host_driven"]["s_per_gradient"] / t_bgrad)
     return out
 
 
@@ -356,12 +380,15 @@ def main(argv=None):
                     help="time the reference-structured single-shot FWI against the batched inversion "
                          "to equal misfit (defaults: 256x256x128, 8 shots, 600 steps)")
     ap.add_argument("--plain-iters", type=int, default=5)
+    ap.add_argument("--no-host-loop", action="store_true",
+                    help="skip timing the plain gradient driven one step at a time from the host")
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
     if args.compare_plain:
         print(json.dumps(compare_plain(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt,
-                                       plain_iters=args.plain_iters, iters=args.iters, log=log)))
+                                       plain_iters=args.plain_iters, iters=args.iters, log=log,
+                                       host_loop=not args.no_host_loop)))
         return
     wt = workloads.c4(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt, anomalies=True)
     w0 = workloads.c4(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt, anomalies=False)

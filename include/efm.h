@@ -226,6 +226,15 @@ int efm_backward_remake_ref(const EfmDims *d, const EfmMedium *m, const EfmShots
                             const EfmState *adj, const float *res, float dt, float dx,
                             float dy, float dz, float *glam, float *gmu, float *grho,
                             void *stream);
+/* Iterations it0 .. it0+nsteps-1 of efm_backward_remake_ref (the body of the
+ * reference's per-step loop, M:1939-2054, which it drives from Python one
+ * iteration at a time).  Ranges that cover 0 .. nt-2 in order give the
+ * launches of one efm_backward_remake_ref call. */
+int efm_backward_remake_ref_steps(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
+                                  const EfmState *fwd, float *vpres, const float *borders, int W,
+                                  const EfmState *adj, const float *res, float dt, float dx,
+                                  float dy, float dz, float *glam, float *gmu, float *grho,
+                                  int it0, int nsteps, void *stream);
 /* Size in floats of the boundary buffer efm_forward writes for one shot. */
 long long efm_border_floats(const EfmDims *d, int W);
 

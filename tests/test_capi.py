@@ -161,6 +161,8 @@ def test_null_shot_arrays_are_refused_before_launch(L, hole):
         "efm_backward_mt_ref": lambda: L.efm_backward_mt_ref(by(d), by(m), by(sh), by(st), 1, 1, 1, None),
         "efm_backward_remake_ref": lambda: L.efm_backward_remake_ref(
             by(d), by(m), by(sh), by(st), 1, 1, 3, by(st), 1, 1.0, 1.0, 1.0, 1.0, 1, 1, 1, None),
+        "efm_backward_remake_ref_steps": lambda: L.efm_backward_remake_ref_steps(
+            by(d), by(m), by(sh), by(st), 1, 1, 3, by(st), 1, 1.0, 1.0, 1.0, 1.0, 1, 1, 1, 0, 1, None),
     }
     fwd = (ctypes.POINTER(type(st)) * 2)(ctypes.pointer(st), ctypes.pointer(st))
     calls["efm_adjoint"] = lambda: L.efm_adjoint(by(d), by(m), 1, 1, by(sh), fwd, by(st), by(st), 1, 1, 1,

@@ -93,6 +93,29 @@ def test_reference_remake_imaging(gpu_lib):
         assert e < 1e-4, (name, e)
 
 
+def test_host_driven_loops_match_the_c_loops(gpu_lib):
+    """host_loop: the reference's structure (one host call per step, each
+    iteration's three images copied to the host and summed there) gives the C
+    loops' gathers bit for bit and their images to float32 summation order."""
+    out = {}
+    for host_loop in (False, True):
+        e3, mdl, wav, cpml, geo = _setup(mode=1, seed=8, nt=90)
+        e3.host_loop = host_loop
+        e3.setIdxEarthquake(0)
+        e3.setStressTensor(D.StressTensor(0.7, 0.2, -0.4, 0.1, 0.3, -0.2))
+        rng = np.random.default_rng(2)
+        e3.obsGather.Vx, e3.obsGather.Vy, e3.obsGather.Vz = (
+            (1e-9 * rng.standard_normal((wav.nSamples, e3.LenStations))).astype(np.float32) for _ in range(3))
+        e3.forwardFields("mod")
+        e3.computeResidual()
+        e3.backwardFields_FowardFieldsRemake()
+        out[host_loop] = (np.stack(e3.getGathers()), e3.gk.Lambda.copy(), e3.gk.Mu.copy(), e3.gk.Rho.copy())
+    np.testing.assert_array_equal(out[True][0], out[False][0])
+    for k in (1, 2, 3):
+        assert np.abs(out[False][k]).max() > 0
+        assert rel_l2(out[True][k], out[False][k]) < 1e-6, k
+
+
 def test_fwi_mij_contract(gpu_lib):
     e3, mdl, wav, cpml, geo = _setup(mode=0, nt=100, seed=7)
     e3.setIdxEarthquake(0)
