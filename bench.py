@@ -34,13 +34,14 @@ HBM_PEAK_GBS = 8000.0
 # traffic excluded):
 #   stress     v×3, λ, μ read; τ×6 read+write                    = 17 words
 #   velocity   τ×6, ρ read; v×3 read+write                        = 13 words
-#   adj_vel    v̄×3, forward τ×6, b̄×3 read; τ̄×6 and ρ-sums×3 r+w   = 30 words
+#   adj_vel    v̄×3, forward v^{n+1}, v^n ×3 each, b̄×3 read; τ̄×6 and
+#              the ρ sum r+w                                       = 26 words
 #   adj_str    τ̄×6, forward v×3, λ, μ, μ̄×3 read; v̄×3, λμ-sums×5 r+w = 30 words
-ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 120, "adj_stress": 120}
+ALG_BYTES = {"stress": 68, "velocity": 52, "adj_velocity": 104, "adj_stress": 120}
 # The byte model of the metric counts an adjoint time step as 120 B (the adjoint
 # state's sweeps) + 24 B (three imaging accumulators read and written) = 144 B
-# per interior cell; the two-sweep design moves 240 B (its τ̄ crosses HBM
-# between the sweeps, it reads the derived medium and keeps 8 accumulators).
+# per interior cell; the two-sweep design moves 224 B (its τ̄ crosses HBM
+# between the sweeps, it reads the derived medium and keeps 6 sums).
 ADJ_STEP_MODEL_BYTES = 144
 FIXTURE_C2 = os.path.join(ROOT, "tests", "golden", "c2_gathers_400.npz")
 
@@ -67,20 +68,18 @@ MEASURED_TRAFFIC = {
         "64,1,1,16": 124630500,
     },
     "adj_velocity": {
-        "128,1,1,0": 361978527, "128,2,1,0": 318361448, "64,1,1,0": 391026011,
-        "64,2,1,0": 347621207, "64,4,1,0": 327157837, "64,4,2,0": 373229361, "128,1,1,8": 298551361,
-        "128,2,1,8": 296304217, "256,1,1,8": 298299512, "64,1,1,8": 298558144,
-        "64,2,1,8": 298189700, "64,4,1,8": 296961211, "64,4,2,8": 307501377, "64,8,1,8": 297139848,
-        "128,1,1,16": 324309946, "128,1,2,16": 343366564, "256,1,1,16": 321780098,
-        "64,1,1,16": 325192463,
+        "128,1,1,0": 297511826, "128,2,1,0": 260657587, "64,1,1,0": 320620904, "64,2,1,0": 283152256,
+        "64,4,1,0": 265897322, "64,4,2,0": 315665916, "128,1,1,8": 231482275, "128,2,1,8": 231144673,
+        "256,1,1,8": 230520839, "64,1,1,8": 231706207, "64,2,1,8": 231522752, "64,4,1,8": 231177726,
+        "64,4,2,8": 260125364, "64,8,1,8": 231440567, "128,1,1,16": 233387198, "128,1,2,16": 273224700,
+        "256,1,1,16": 232906791, "64,1,1,16": 233645634,
     },
     "adj_stress": {
-        "128,1,1,0": 387067637, "128,2,1,0": 331348397, "64,1,1,0": 422321657,
-        "64,2,1,0": 368304597, "64,4,1,0": 341271877, "64,4,2,0": 409287811, "128,1,1,8": 303227882,
-        "128,2,1,8": 301568936, "256,1,1,8": 301902310, "64,1,1,8": 303021821,
-        "64,2,1,8": 302194991, "64,4,1,8": 303146738, "64,4,2,8": 333767018, "64,8,1,8": 302700156,
-        "128,1,1,16": 322531059, "128,1,2,16": 363893777, "256,1,1,16": 320776489,
-        "64,1,1,16": 323266959,
+        "128,1,1,0": 388909521, "128,2,1,0": 333243940, "64,1,1,0": 423455803, "64,2,1,0": 369612415,
+        "64,4,1,0": 343047450, "64,4,2,0": 409583004, "128,1,1,8": 302100609, "128,2,1,8": 301023487,
+        "256,1,1,8": 301078906, "64,1,1,8": 302183753, "64,2,1,8": 302240453, "64,4,1,8": 302374969,
+        "64,4,2,8": 332926850, "64,8,1,8": 302210872, "128,1,1,16": 322663382, "128,1,2,16": 363389368,
+        "256,1,1,16": 320352764, "64,1,1,16": 323488295,
     },
 }
 

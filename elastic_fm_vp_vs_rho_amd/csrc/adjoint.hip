@@ -39,7 +39,8 @@ struct AdjArgs {
     StatePtrs a, a2;               // adjoint: a = live (v̄, τ̄, C in), a2 = C out
     float *acc;                    // per shot 8 padded components: λ, μ, μ_xy, μ_xz sums
                                    // interleaved (one float4 per cell) in components 0-3,
-                                   // the ρ_x, ρ_y, ρ_z sums interleaved in 4-6, μ_yz in 7
+                                   // the ρ sum in 4, μ_yz in 7 (5 and 6 unused: 8 components
+                                   // keep every shot's float4 base 16-byte aligned for any N)
     ShotPtrs sh;
     const float *resid;            // ∂J/∂gathers, used by R4 when the shots carry a receiver mask
     double *gm6;                   // ∂J/∂M, likewise
@@ -90,7 +91,8 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
     float *__restrict__ tb4 = A.a.t[4] + so;
     float *__restrict__ tb5 = A.a.t[5] + so;
     // forward velocities after and before step n: their difference is c·b̄·F, the
-    // velocity update the ρ accumulators need (b̄ and c are applied by k_combine)
+    // velocity update the ρ products need (read at the cell and at p-x, p-y, p-z, so
+    // without the streaming hint)
     const float *__restrict__ Wx1 = A.fn1.v[0] + so;
     const float *__restrict__ Wy1 = A.fn1.v[1] + so;
     const float *__restrict__ Wz1 = A.fn1.v[2] + so;
@@ -103,7 +105,7 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
     float *__restrict__ Cxo = A.a2.px + px_shot(g, s);
     float *__restrict__ Cyo = A.a2.py + py_shot(g, s);
     float *__restrict__ Czo = A.a2.pz + pz_shot(g, s);
-    // ρ sums: 3 consecutive floats per cell at component offset 4 of the shot
+    // ρ sums: component 4 of the shot
     float *__restrict__ accr = A.acc + ((long long)s * 8 + 4) * g.ps;
 
     for (int y = y0; y < y1; ++y) {
@@ -186,15 +188,22 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
         tb4[t] += (hb_xzz_0 - hb_xzz_p) + (hb_xzx_0 - hb_xzx_p);
         tb5[t] += (hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p);
 
-        // ρ accumulators: v̄ times this step's forward velocity change
-        const float dwx = LDF(&Wx1[t]) - LDF(&Wx0[t]);
-        const float dwy = LDF(&Wy1[t]) - LDF(&Wy0[t]);
-        const float dwz = LDF(&Wz1[t]) - LDF(&Wz0[t]);
-        float *pr = accr + 3 * (long long)t;
-        const float r0 = pr[0], r1 = pr[1], r2 = pr[2];
-        pr[0] = r0 + vx0 * dwx;
-        pr[1] = r1 + vy0 * dwy;
-        pr[2] = r2 + vz0 * dwz;
+        // ρ: this cell enters the averaged buoyancy of its own three faces and of the
+        // faces at p-x, p-y, p-z; each face contributes u·Δv (= c·b̄·v̄ times this step's
+        // forward velocity change).  u vanishes off the interior (b̄ = 0 there).  A face
+        // on the last interior plane also feeds the cell beyond it, which no thread of
+        // this sweep owns; that cell has no other face in the interior, so the face's
+        // thread adds it there (one writer per cell).
+        const float px = ux * (Wx1[t] - Wx0[t]);
+        const float py = uy * (Wy1[t] - Wy0[t]);
+        const float pz = uz * (Wz1[t] - Wz0[t]);
+        const float pxm = ux_mx * (Wx1[t - 1] - Wx0[t - 1]);
+        const float pym = uy_my * (Wy1[t - sxz] - Wy0[t - sxz]);
+        const float pzm = uz_mz * (Wz1[t - nx] - Wz0[t - nx]);
+        accr[t] += (px + pxm) + (py + pym) + (pz + pzm);
+        if (x == g.nx - H - 1) accr[t + 1] += px;
+        if (y == g.ny - H - 1) accr[t + sxz] += py;
+        if (z == g.nz - H - 1) accr[t + nx] += pz;
     }
 }
 
@@ -498,15 +507,12 @@ __global__ void k_combine(Geo g, const float *mu, const float *mbar, const float
     const long long N = g.ps, nx = g.nx, sxz = g.sxz;   // N: padded component stride
     const float c = g.c;
     const float *mxy = mbar, *mxz = mbar + N, *myz = mbar + 2 * N;
-    const float *bbx = bbar, *bby = bbar + N, *bbz = bbar + 2 * N;
+    (void)bbar;                   // the ρ sums already carry b̄
     float sl = 0.f, sm = 0.f, shear = 0.f, sr = 0.f;
     for (int s = 0; s < g.S; ++s) {
         const float *A = acc + (long long)s * 8 * N;
-        // component k of cell j: 0-3 interleaved at the base, 4 (μ_yz) at 7·N,
-        // 5-7 (ρ) interleaved at 4·N
-        auto AC = [&](int k, long long j) -> float {
-            return k < 4 ? A[4 * j + k] : k == 4 ? A[7 * N + j] : A[4 * N + 3 * j + (k - 5)];
-        };
+        // component k of cell j: 0-3 interleaved at the base, 4 (μ_yz) at 7·N
+        auto AC = [&](int k, long long j) -> float { return k < 4 ? A[4 * j + k] : A[7 * N + j]; };
         sl += AC(0, i);
         sm += AC(1, i);
         auto sh4 = [&](int comp, const float *mb, long long o1, bool v1, long long o2, bool v2) {
@@ -519,18 +525,14 @@ __global__ void k_combine(Geo g, const float *mu, const float *mbar, const float
         shear += sh4(2, mxy, 1, x > 0, sxz, y > 0);
         shear += sh4(3, mxz, 1, x > 0, nx, z > 0);
         shear += sh4(4, myz, sxz, y > 0, nx, z > 0);
-        // ρ sums hold Σ v̄·Δv = c·b̄·Σ v̄·F; ∂J/∂ρ = -(c/2)·b̄²·Σ v̄·F = -(1/2)·b̄·Σ v̄·Δv
-        // at both cells each averaged buoyancy couples
-        float r = AC(5, i) * bbx[i] + AC(6, i) * bby[i] + AC(7, i) * bbz[i];
-        if (x > 0) r += AC(5, i - 1) * bbx[i - 1];
-        if (y > 0) r += AC(6, i - sxz) * bby[i - sxz];
-        if (z > 0) r += AC(7, i - nx) * bbz[i - nx];
-        sr += r;
+        // the ρ sum holds Σ c·b̄·v̄·Δv over the cell's six faces; per face
+        // ∂J/∂ρ = -(c/2)·b̄²·v̄·F = -(1/(2c))·(c·b̄·v̄·Δv)
+        sr += A[4 * N + i];
     }
     const float m = mu[i];
     glam[i] = c * sl;
     gmu[i] = c * (sm + (m != 0.f ? 0.25f * shear / (m * m) : 0.f));
-    grho[i] = -0.5f * sr;
+    grho[i] = -0.5f / c * sr;
 }
 
 // r = mod - obs and J += ½Σr² (block partial sums in double, one atomic per block)

@@ -169,9 +169,10 @@ int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float 
  * adj/adj2 are the adjoint state and a second ψ buffer (ping-pong; on return
  * the live ψ adjoint is in adj).  resid is ∂J/∂gathers (S, 3, ntmax, nrec).
  * acc (S·8 components of stride N + EFM_PAD, zeroed by the caller) accumulates
- * the λ, μ_n, μ_xy, μ_xz sums (interleaved per cell in components 0-3), the
- * ρx, ρy, ρz sums (interleaved in components 4-6) and the μ_yz sum (component
- * 7) of each shot; only efm_gradient_combine reads it.  gm6 (S, 6) double
+ * the λ, μ_n, μ_xy, μ_xz sums (interleaved per cell in components 0-3), the ρ
+ * sum (component 4; it also covers the cells one past the interior on the high
+ * x, y, z faces) and the μ_yz sum (component 7) of each shot; components 5
+ * and 6 are not used.  Only efm_gradient_combine reads it.  gm6 (S, 6) double
  * accumulates ∂J/∂M.  The run is
  * covered by calls over consecutive descending ranges, the first with
  * n1 = ntmax.  Without a receiver mask `last` = 1 marks the call with n0 = 0
