@@ -52,34 +52,32 @@ FIXTURE_C2 = os.path.join(ROOT, "tests", "golden", "c2_gathers_400.npz")
 # the 128^3 grid; any other size reports traffic null.
 MEASURED_TRAFFIC = {
     "stress": {
-        "128,1,1,0": 191197943, "128,2,1,0": 171725286, "64,1,1,0": 208426950,
-        "64,2,1,0": 187274175, "64,4,1,0": 177761141, "64,4,2,0": 182717143, "128,1,1,8": 155811077,
-        "128,2,1,8": 155800902, "256,1,1,8": 155797702, "64,1,1,8": 155801414,
-        "64,2,1,8": 155800678, "64,4,1,8": 155801030, "64,4,2,8": 155900573, "64,8,1,8": 155785799,
-        "128,1,1,16": 155906500, "128,1,2,16": 157177212, "256,1,1,16": 155881668,
-        "64,1,1,16": 155922244,
+        "128,1,1,0": 191204485, "128,2,1,0": 171725521, "64,1,1,0": 208426334, "64,2,1,0": 187269119,
+        "64,4,1,0": 177755618, "64,4,2,0": 182869281, "128,1,1,8": 155820708, "128,2,1,8": 155812324,
+        "256,1,1,8": 155798693, "64,1,1,8": 155818212, "64,2,1,8": 155816388, "64,4,1,8": 155817796,
+        "64,4,2,8": 155963415, "64,8,1,8": 155785351, "128,1,1,16": 155892740, "128,1,2,16": 157123737,
+        "256,1,1,16": 155876225, "64,1,1,16": 155889154,
     },
     "velocity": {
-        "128,1,1,0": 159635284, "128,2,1,0": 140184002, "64,1,1,0": 174886531,
-        "64,2,1,0": 154752186, "64,4,1,0": 145709634, "64,4,2,0": 148359267, "128,1,1,8": 124559653,
-        "128,2,1,8": 124300094, "256,1,1,8": 124292671, "64,1,1,8": 124299646,
-        "64,2,1,8": 124299198, "64,4,1,8": 124292031, "64,4,2,8": 124388374, "64,8,1,8": 124299198,
-        "128,1,1,16": 124346493, "128,1,2,16": 125936803, "256,1,1,16": 124316286,
-        "64,1,1,16": 124630500,
+        "128,1,1,0": 159640317, "128,2,1,0": 140184295, "64,1,1,0": 174888005, "64,2,1,0": 154752491,
+        "64,4,1,0": 145709829, "64,4,2,0": 148141615, "128,1,1,8": 124301337, "128,2,1,8": 124300537,
+        "256,1,1,8": 124293209, "64,1,1,8": 124300889, "64,2,1,8": 124300697, "64,4,1,8": 124551203,
+        "64,4,2,8": 124400208, "64,8,1,8": 124299961, "128,1,1,16": 124605633, "128,1,2,16": 125901982,
+        "256,1,1,16": 124314872, "64,1,1,16": 124392695,
     },
     "adj_velocity": {
-        "128,1,1,0": 297511826, "128,2,1,0": 260657587, "64,1,1,0": 320620904, "64,2,1,0": 283152256,
-        "64,4,1,0": 265897322, "64,4,2,0": 315665916, "128,1,1,8": 231482275, "128,2,1,8": 231144673,
-        "256,1,1,8": 230520839, "64,1,1,8": 231706207, "64,2,1,8": 231522752, "64,4,1,8": 231177726,
-        "64,4,2,8": 260125364, "64,8,1,8": 231440567, "128,1,1,16": 233387198, "128,1,2,16": 273224700,
-        "256,1,1,16": 232906791, "64,1,1,16": 233645634,
+        "128,1,1,0": 297545856, "128,2,1,0": 260632517, "64,1,1,0": 320705185, "64,2,1,0": 283115923,
+        "64,4,1,0": 265872956, "64,4,2,0": 316756892, "128,1,1,8": 231823206, "128,2,1,8": 231690225,
+        "256,1,1,8": 230632702, "64,1,1,8": 232065984, "64,2,1,8": 231876529, "64,4,1,8": 231833749,
+        "64,4,2,8": 257150123, "64,8,1,8": 230948002, "128,1,1,16": 235673336, "128,1,2,16": 272388325,
+        "256,1,1,16": 234971698, "64,1,1,16": 235722217,
     },
     "adj_stress": {
-        "128,1,1,0": 388909521, "128,2,1,0": 333243940, "64,1,1,0": 423455803, "64,2,1,0": 369612415,
-        "64,4,1,0": 343047450, "64,4,2,0": 409583004, "128,1,1,8": 302100609, "128,2,1,8": 301023487,
-        "256,1,1,8": 301078906, "64,1,1,8": 302183753, "64,2,1,8": 302240453, "64,4,1,8": 302374969,
-        "64,4,2,8": 332926850, "64,8,1,8": 302210872, "128,1,1,16": 322663382, "128,1,2,16": 363389368,
-        "256,1,1,16": 320352764, "64,1,1,16": 323488295,
+        "128,1,1,0": 388572528, "128,2,1,0": 333250435, "64,1,1,0": 422697662, "64,2,1,0": 369768114,
+        "64,4,1,0": 342773314, "64,4,2,0": 474483390, "128,1,1,8": 303909027, "128,2,1,8": 302263763,
+        "256,1,1,8": 302182409, "64,1,1,8": 304887245, "64,2,1,8": 304349165, "64,4,1,8": 303555698,
+        "64,4,2,8": 412925447, "64,8,1,8": 302398056, "128,1,1,16": 325934616, "128,1,2,16": 468485086,
+        "256,1,1,16": 322644611, "64,1,1,16": 326988847,
     },
 }
 

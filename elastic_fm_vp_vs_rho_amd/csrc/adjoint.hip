@@ -32,6 +32,14 @@ namespace efm {
 // the reused adjoint state, medium and accumulators cached.
 #define LDF(p) __builtin_nontemporal_load(p)
 
+// CPML coefficient of cell i (0 outside the slab, slab coordinate < 0).  The
+// load is unconditional (every interior cell's ±1 neighbour indexes the
+// profile), so it issues with the others instead of behind a branch.
+static __device__ __forceinline__ float coef(const float *p, int i, int slab) {
+    const float v = p[i];
+    return slab >= 0 ? v : 0.f;
+}
+
 struct AdjArgs {
     const float *lam, *mu;
     const float *mbar, *bbar;      // (3, N) each
@@ -72,10 +80,10 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
     const float *axs = sh.ax + (long long)s * g.nx, *bxs = sh.bx + (long long)s * g.nx;
     const float *ays = sh.ay + (long long)s * g.ny, *bys = sh.by + (long long)s * g.ny;
     const float *azs = sh.az + (long long)s * g.nz, *bzs = sh.bz + (long long)s * g.nz;
-    const float ax0 = sx0 >= 0 ? axs[x] : 0.f, bx0 = sx0 >= 0 ? bxs[x] : 0.f;
-    const float axm = sxm >= 0 ? axs[x - 1] : 0.f, axp = sxp >= 0 ? axs[x + 1] : 0.f;
-    const float az0 = sz0 >= 0 ? azs[z] : 0.f, bz0 = sz0 >= 0 ? bzs[z] : 0.f;
-    const float azm = szm >= 0 ? azs[z - 1] : 0.f, azp = szp >= 0 ? azs[z + 1] : 0.f;
+    const float ax0 = coef(axs, x, sx0), bx0 = coef(bxs, x, sx0);
+    const float axm = coef(axs, x - 1, sxm), axp = coef(axs, x + 1, sxp);
+    const float az0 = coef(azs, z, sz0), bz0 = coef(bzs, z, sz0);
+    const float azm = coef(azs, z - 1, szm), azp = coef(azs, z + 1, szp);
     const int csx = g.pxc, csy = g.pyc, csz = g.pzc;
 
     const float *__restrict__ bbx = A.bbar;
@@ -113,8 +121,8 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
         const int sy0 = slab_coord(y, g.ny, g.wyl, g.wyh);
         const int sym = slab_coord(y - 1, g.ny, g.wyl, g.wyh);
         const int syp = slab_coord(y + 1, g.ny, g.wyl, g.wyh);
-        const float ay0 = sy0 >= 0 ? ays[y] : 0.f, by0 = sy0 >= 0 ? bys[y] : 0.f;
-        const float aym = sym >= 0 ? ays[y - 1] : 0.f, ayp = syp >= 0 ? ays[y + 1] : 0.f;
+        const float ay0 = coef(ays, y, sy0), by0 = coef(bys, y, sy0);
+        const float aym = coef(ays, y - 1, sym), ayp = coef(ays, y + 1, syp);
 
         // u = c·b̄·v̄ at p and at the neighbours the transposes need
         const float vx0 = vbx[t], vy0 = vby[t], vz0 = vbz[t];
@@ -210,8 +218,11 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
 // ---------------------------------------------------------------------------
 // R4: transpose of the stress sweep
 // ---------------------------------------------------------------------------
+// At least 7 waves per SIMD: the compiler then fits the sweep in 71 VGPRs (73 by
+// itself, which allows only 6) by moving some SGPRs to VGPR lanes; the C2 adjoint
+// step gets 2-3 % faster.
 template <int BX, int BZ, int NYC, int MAP>
-__global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_stress(Geo g, AdjArgs A, int n) {
+__global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_adj_stress(Geo g, AdjArgs A, int n) {
     int x, z, s, yb;
     if (!tile_coords<BX, BZ, NYC, MAP>(g, x, z, s, yb)) return;
     if (x < H || x >= g.nx - H || z < H || z >= g.nz - H) return;
