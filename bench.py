@@ -52,32 +52,32 @@ FIXTURE_C2 = os.path.join(ROOT, "tests", "golden", "c2_gathers_400.npz")
 # the 128^3 grid; any other size reports traffic null.
 MEASURED_TRAFFIC = {
     "stress": {
-        "128,1,1,0": 191204485, "128,2,1,0": 171725521, "64,1,1,0": 208426334, "64,2,1,0": 187269119,
-        "64,4,1,0": 177755618, "64,4,2,0": 182869281, "128,1,1,8": 155820708, "128,2,1,8": 155812324,
-        "256,1,1,8": 155798693, "64,1,1,8": 155818212, "64,2,1,8": 155816388, "64,4,1,8": 155817796,
-        "64,4,2,8": 155963415, "64,8,1,8": 155785351, "128,1,1,16": 155892740, "128,1,2,16": 157123737,
-        "256,1,1,16": 155876225, "64,1,1,16": 155889154,
+        "128,1,1,0": 191197637, "128,2,1,0": 171724689, "64,1,1,0": 208424926, "64,2,1,0": 187271551,
+        "64,4,1,0": 177759425, "64,4,2,0": 183068655, "128,1,1,8": 155800165, "128,2,1,8": 155800773,
+        "256,1,1,8": 155798693, "64,1,1,8": 155800069, "64,2,1,8": 155797926, "64,4,1,8": 155799461,
+        "64,4,2,8": 156048560, "64,8,1,8": 155785703, "128,1,1,16": 155861316, "128,1,2,16": 157314835,
+        "256,1,1,16": 155860004, "64,1,1,16": 155954500,
     },
     "velocity": {
-        "128,1,1,0": 159640317, "128,2,1,0": 140184295, "64,1,1,0": 174888005, "64,2,1,0": 154752491,
-        "64,4,1,0": 145709829, "64,4,2,0": 148141615, "128,1,1,8": 124301337, "128,2,1,8": 124300537,
-        "256,1,1,8": 124293209, "64,1,1,8": 124300889, "64,2,1,8": 124300697, "64,4,1,8": 124551203,
-        "64,4,2,8": 124400208, "64,8,1,8": 124299961, "128,1,1,16": 124605633, "128,1,2,16": 125901982,
-        "256,1,1,16": 124314872, "64,1,1,16": 124392695,
+        "128,1,1,0": 161603185, "128,2,1,0": 140184167, "64,1,1,0": 174893508, "64,2,1,0": 154752523,
+        "64,4,1,0": 145708997, "64,4,2,0": 148252102, "128,1,1,8": 124300601, "128,2,1,8": 124302857,
+        "256,1,1,8": 124293337, "64,1,1,8": 124300505, "64,2,1,8": 124301241, "64,4,1,8": 124293081,
+        "64,4,2,8": 124414799, "64,8,1,8": 124302617, "128,1,1,16": 124601857, "128,1,2,16": 125905375,
+        "256,1,1,16": 124322808, "64,1,1,16": 124376568,
     },
     "adj_velocity": {
-        "128,1,1,0": 297545856, "128,2,1,0": 260632517, "64,1,1,0": 320705185, "64,2,1,0": 283115923,
-        "64,4,1,0": 265872956, "64,4,2,0": 316756892, "128,1,1,8": 231823206, "128,2,1,8": 231690225,
-        "256,1,1,8": 230632702, "64,1,1,8": 232065984, "64,2,1,8": 231876529, "64,4,1,8": 231833749,
-        "64,4,2,8": 257150123, "64,8,1,8": 230948002, "128,1,1,16": 235673336, "128,1,2,16": 272388325,
-        "256,1,1,16": 234971698, "64,1,1,16": 235722217,
+        "128,1,1,0": 297996656, "128,2,1,0": 260773961, "64,1,1,0": 321601941, "64,2,1,0": 283630696,
+        "64,4,1,0": 265993074, "64,4,2,0": 312446312, "128,1,1,8": 231463820, "128,2,1,8": 231460153,
+        "256,1,1,8": 230669693, "64,1,1,8": 231400148, "64,2,1,8": 231437298, "64,4,1,8": 231500798,
+        "64,4,2,8": 253276609, "64,8,1,8": 232402444, "128,1,1,16": 239307653, "128,1,2,16": 266988764,
+        "256,1,1,16": 237318627, "64,1,1,16": 239244164,
     },
     "adj_stress": {
-        "128,1,1,0": 388572528, "128,2,1,0": 333250435, "64,1,1,0": 422697662, "64,2,1,0": 369768114,
-        "64,4,1,0": 342773314, "64,4,2,0": 474483390, "128,1,1,8": 303909027, "128,2,1,8": 302263763,
-        "256,1,1,8": 302182409, "64,1,1,8": 304887245, "64,2,1,8": 304349165, "64,4,1,8": 303555698,
-        "64,4,2,8": 412925447, "64,8,1,8": 302398056, "128,1,1,16": 325934616, "128,1,2,16": 468485086,
-        "256,1,1,16": 322644611, "64,1,1,16": 326988847,
+        "128,1,1,0": 388869317, "128,2,1,0": 333875468, "64,1,1,0": 422968550, "64,2,1,0": 370303011,
+        "64,4,1,0": 343344816, "64,4,2,0": 474192917, "128,1,1,8": 304226871, "128,2,1,8": 302532747,
+        "256,1,1,8": 302041446, "64,1,1,8": 305047935, "64,2,1,8": 304884462, "64,4,1,8": 304141087,
+        "64,4,2,8": 412439331, "64,8,1,8": 302609413, "128,1,1,16": 327087078, "128,1,2,16": 468075640,
+        "256,1,1,16": 323230582, "64,1,1,16": 327790592,
     },
 }
 

@@ -124,6 +124,16 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
         const float ay0 = coef(ays, y, sy0), by0 = coef(bys, y, sy0);
         const float aym = coef(ays, y - 1, sym), ayp = coef(ays, y + 1, syp);
 
+        // Every load of the cell that does not depend on the CPML slabs is issued
+        // here, before the first store: the compiler cannot prove that the stores
+        // below miss these arrays, so a load placed after a store would wait for it.
+        const float tbo0 = tb0[t], tbo1 = tb1[t], tbo2 = tb2[t];
+        const float tbo3 = tb3[t], tbo4 = tb4[t], tbo5 = tb5[t];
+        const float acco = accr[t];
+        const float dwx = Wx1[t] - Wx0[t], dwy = Wy1[t] - Wy0[t], dwz = Wz1[t] - Wz0[t];
+        const float dwx_m = Wx1[t - 1] - Wx0[t - 1];
+        const float dwy_m = Wy1[t - sxz] - Wy0[t - sxz];
+        const float dwz_m = Wz1[t - nx] - Wz0[t - nx];
         // u = c·b̄·v̄ at p and at the neighbours the transposes need
         const float vx0 = vbx[t], vy0 = vby[t], vz0 = vbz[t];
         const float ux = c * bbx[t] * vx0;
@@ -138,6 +148,11 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
         const float uz_px = c * bbz[t + 1] * vbz[t + 1];        // txz_x at p+x
         const float uy_pz = c * bby[t + nx] * vby[t + nx];      // tyz_z at p+z
         const float uz_py = c * bbz[t + sxz] * vbz[t + sxz];    // tyz_y at p+y
+        // ρ: this cell enters the averaged buoyancy of its own three faces and of the
+        // faces at p-x, p-y, p-z; each face contributes u·Δv (= c·b̄·v̄ times this step's
+        // forward velocity change).  u vanishes off the interior (b̄ = 0 there).
+        const float px = ux * dwx, py = uy * dwy, pz = uz * dwz;
+        const float accn = acco + ((px + ux_mx * dwx_m) + (py + uy_my * dwy_m) + (pz + uz_mz * dwz_m));
 
         // h̄ = u + a(C + u) with C read from the "in" buffer
         float hb_xx_m = ux_mx, hb_xx_0 = ux;           // txx_x
@@ -189,26 +204,17 @@ __global__ EFM_KERNEL_ATTRS(BX, BZ, MAP) void k_adj_velocity(Geo g, AdjArgs A, i
             Czo[pi] = bz0 * c0; Czo[pi + csz] = bz0 * c1; Czo[pi + 2 * csz] = bz0 * c2;
         }
         // τ̄ += D_τᵀ h̄
-        tb0[t] += hb_xx_m - hb_xx_0;
-        tb1[t] += hb_yy_m - hb_yy_0;
-        tb2[t] += hb_zz_m - hb_zz_0;
-        tb3[t] += (hb_xyy_0 - hb_xyy_p) + (hb_xyx_0 - hb_xyx_p);
-        tb4[t] += (hb_xzz_0 - hb_xzz_p) + (hb_xzx_0 - hb_xzx_p);
-        tb5[t] += (hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p);
+        tb0[t] = tbo0 + (hb_xx_m - hb_xx_0);
+        tb1[t] = tbo1 + (hb_yy_m - hb_yy_0);
+        tb2[t] = tbo2 + (hb_zz_m - hb_zz_0);
+        tb3[t] = tbo3 + ((hb_xyy_0 - hb_xyy_p) + (hb_xyx_0 - hb_xyx_p));
+        tb4[t] = tbo4 + ((hb_xzz_0 - hb_xzz_p) + (hb_xzx_0 - hb_xzx_p));
+        tb5[t] = tbo5 + ((hb_yzz_0 - hb_yzz_p) + (hb_yzy_0 - hb_yzy_p));
 
-        // ρ: this cell enters the averaged buoyancy of its own three faces and of the
-        // faces at p-x, p-y, p-z; each face contributes u·Δv (= c·b̄·v̄ times this step's
-        // forward velocity change).  u vanishes off the interior (b̄ = 0 there).  A face
-        // on the last interior plane also feeds the cell beyond it, which no thread of
-        // this sweep owns; that cell has no other face in the interior, so the face's
-        // thread adds it there (one writer per cell).
-        const float px = ux * (Wx1[t] - Wx0[t]);
-        const float py = uy * (Wy1[t] - Wy0[t]);
-        const float pz = uz * (Wz1[t] - Wz0[t]);
-        const float pxm = ux_mx * (Wx1[t - 1] - Wx0[t - 1]);
-        const float pym = uy_my * (Wy1[t - sxz] - Wy0[t - sxz]);
-        const float pzm = uz_mz * (Wz1[t - nx] - Wz0[t - nx]);
-        accr[t] += (px + pxm) + (py + pym) + (pz + pzm);
+        // A face on the last interior plane also feeds the cell beyond it, which no
+        // thread of this sweep owns; that cell has no other face in the interior, so
+        // the face's thread adds it there (one writer per cell).
+        accr[t] = accn;
         if (x == g.nx - H - 1) accr[t + 1] += px;
         if (y == g.ny - H - 1) accr[t + sxz] += py;
         if (z == g.nz - H - 1) accr[t + nx] += pz;
