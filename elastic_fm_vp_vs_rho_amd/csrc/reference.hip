@@ -71,58 +71,70 @@ __global__ void k_image_mt_ref(Geo g, const float *lam, const float *mu, ShotPtr
 }
 
 // λ/μ/ρ imaging conditions, accumulated in float32 (the reference adds the
-// per-step device volumes on the host in float32).
+// per-step device volumes on the host in float32).  Divisions by the medium
+// use the hardware reciprocal (1 ulp) and those by dt, dx, dy, dz the
+// reciprocals of the steps: the images move by a few ulp, far inside the
+// 1e-4 the gradients are held to.  Index arithmetic is 32-bit (N < 2^31).
 __global__ void k_image_ref(Geo g, const float *lam, const float *mu, StatePtrs a, StatePtrs f,
                             StatePtrs fp, float dt, float dx, float dy, float dz, float *glam,
                             float *gmu, float *grho) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
     if (i >= g.N) return;
-    const int x = (int)(i % g.nx);
-    const int z = (int)((i / g.nx) % g.nz);
-    const int y = (int)(i / g.sxz);
+    const int row = i / g.nx;
+    const int x = i - row * g.nx;
+    const int y = row / g.nz;
+    const int z = row - y * g.nz;
     if (!interior(x, y, z, g)) return;
-    const long long nx = g.nx, sxz = g.sxz;
-    const long long t = (long long)s * g.N + i;
+    const int nx = g.nx, sxz = g.sxz32;
+    const long long so = (long long)s * g.N;
+    const float *__restrict__ av0 = a.v[0] + so, *__restrict__ av1 = a.v[1] + so, *__restrict__ av2 = a.v[2] + so;
+    const float *__restrict__ vx = fp.v[0] + so, *__restrict__ vy = fp.v[1] + so, *__restrict__ vz = fp.v[2] + so;
+    const float *__restrict__ fx = f.v[0] + so, *__restrict__ fy = f.v[1] + so, *__restrict__ fz = f.v[2] + so;
+    const float idt = 1.f / dt, idx = 1.f / dx, idy = 1.f / dy, idz = 1.f / dz;
     // M:206
-    const float r = -((a.v[0][t] * (fp.v[0][t] - f.v[0][t]) / dt) +
-                      (a.v[1][t] * (fp.v[1][t] - f.v[1][t]) / dt) +
-                      (a.v[2][t] * (fp.v[2][t] - f.v[2][t]) / dt));
-    const float *vx = fp.v[0], *vy = fp.v[1], *vz = fp.v[2];
+    const float r = -((av0[i] * (vx[i] - fx[i]) * idt) + (av1[i] * (vy[i] - fy[i]) * idt) +
+                      (av2[i] * (vz[i] - fz[i]) * idt));
     const float L = lam[i], M = mu[i];
     // M:234-238
-    const float dvx = 0.5f * (vx[t] - vx[t - 1]) / dx + 0.5f * (vx[t - 1] - vx[t - 2]) / dx;
-    const float dvy = 0.5f * (vy[t] - vy[t - sxz]) / dy + 0.5f * (vy[t - sxz] - vy[t - 2 * sxz]) / dy;
-    const float dvz = 0.5f * (vz[t] - vz[t - nx]) / dz + 0.5f * (vz[t - nx] - vz[t - 2 * nx]) / dz;
-    const float txx = a.t[0][t], tyy = a.t[1][t], tzz = a.t[2][t];
-    const float gl = (txx + tyy + tzz) * (dvx + dvy + dvz) / (3.f * L + 2.f * M);
-    // M:262-291
-    const float mxz = 1.f / (0.25f / M + 0.25f / mu[i + 1] + 0.25f / mu[i + nx] + 0.25f / mu[i + 1 + nx]);
-    const float mxy = 1.f / (0.25f / M + 0.25f / mu[i + 1] + 0.25f / mu[i + sxz] + 0.25f / mu[i + 1 + sxz]);
-    const float myz = 1.f / (0.25f / M + 0.25f / mu[i + sxz] + 0.25f / mu[i + nx] + 0.25f / mu[i + nx + sxz]);
-    const float ex = vx[t] - vx[t - 1], ey = vy[t] - vy[t - sxz], ez = vz[t] - vz[t - nx];
-    const float dvx_dz = vx[t + nx] - vx[t], dvx_dy = vx[t + sxz] - vx[t];
-    const float dvy_dx = vy[t + 1] - vy[t], dvy_dz = vy[t + nx] - vy[t];
-    const float dvz_dx = vz[t + 1] - vz[t], dvz_dy = vz[t + sxz] - vz[t];
-    const float k = 1.f / (M * (3.f * L + 2.f * M));
+    const float dvx = 0.5f * (vx[i] - vx[i - 1]) * idx + 0.5f * (vx[i - 1] - vx[i - 2]) * idx;
+    const float dvy = 0.5f * (vy[i] - vy[i - sxz]) * idy + 0.5f * (vy[i - sxz] - vy[i - 2 * sxz]) * idy;
+    const float dvz = 0.5f * (vz[i] - vz[i - nx]) * idz + 0.5f * (vz[i - nx] - vz[i - 2 * nx]) * idz;
+    const float txx = a.t[0][so + i], tyy = a.t[1][so + i], tzz = a.t[2][so + i];
+    const float k3 = frcp(3.f * L + 2.f * M);
+    const float gl = (txx + tyy + tzz) * (dvx + dvy + dvz) * k3;
+    // M:262-291: 1/μ̄ for the three shear planes (the reference inverts the
+    // harmonic mean and then inverts it back; here the sum of quarter
+    // reciprocals is used directly)
+    const float q0 = 0.25f * frcp(M), qi = 0.25f * frcp(mu[i + 1]), qk = 0.25f * frcp(mu[i + nx]);
+    const float qj = 0.25f * frcp(mu[i + sxz]);
+    const float imxz = q0 + qi + qk + 0.25f * frcp(mu[i + 1 + nx]);
+    const float imxy = q0 + qi + qj + 0.25f * frcp(mu[i + 1 + sxz]);
+    const float imyz = q0 + qj + qk + 0.25f * frcp(mu[i + nx + sxz]);
+    const float ex = vx[i] - vx[i - 1], ey = vy[i] - vy[i - sxz], ez = vz[i] - vz[i - nx];
+    const float dvx_dz = vx[i + nx] - vx[i], dvx_dy = vx[i + sxz] - vx[i];
+    const float dvy_dx = vy[i + 1] - vy[i], dvy_dz = vy[i + nx] - vy[i];
+    const float dvz_dx = vz[i + 1] - vz[i], dvz_dy = vz[i + sxz] - vz[i];
+    const float k = frcp(M) * k3;
     const float gm = k * (2.f * (L + M) * txx - L * tyy - L * tzz) * ex
                    + k * (2.f * (L + M) * tyy - L * txx - L * tzz) * ey
                    + k * (2.f * (L + M) * tzz - L * txx - L * tyy) * ez
-                   + (1.f / mxy) * a.t[3][t] * (dvy_dx + dvx_dy)
-                   + (1.f / mxz) * a.t[4][t] * (dvx_dz + dvz_dx)
-                   + (1.f / myz) * a.t[5][t] * (dvz_dy + dvy_dz);
-    glam[t] += gl;
-    gmu[t] += gm;
-    grho[t] += r;
+                   + imxy * a.t[3][so + i] * (dvy_dx + dvx_dy)
+                   + imxz * a.t[4][so + i] * (dvx_dz + dvz_dx)
+                   + imyz * a.t[5][so + i] * (dvz_dy + dvy_dz);
+    glam[so + i] += gl;
+    gmu[so + i] += gm;
+    grho[so + i] += r;
 }
 
 __global__ void k_zero_cpml(Geo g, StatePtrs f, int W) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
     if (i >= g.N) return;
-    const int x = (int)(i % g.nx);
-    const int z = (int)((i / g.nx) % g.nz);
-    const int y = (int)(i / g.sxz);
+    const int row = i / g.nx;
+    const int x = i - row * g.nx;
+    const int y = row / g.nz;
+    const int z = row - y * g.nz;
     if (!(x < W || x >= g.nx - W || z >= g.nz - W || y < W || y >= g.ny - W)) return;
     const long long t = (long long)s * g.N + i;
     for (int k = 0; k < 3; ++k) f.v[k][t] = 0.f;
