@@ -6,7 +6,11 @@
 * c4_shot0_300.npz    one shot (shot 0) of the C4 true model at full size
                       (200×200×100, two Gaussian anomalies), 300 steps;
 * c5_two_shots.npz    the first two shots of the C5 workload (96³ random
-                      layered model), 1024 steps each.
+                      layered model), 1024 steps each;
+* c2_gradient_40.npz  the exact FWI gradient (λ, μ, ρ volumes and ∂J/∂M) of the
+                      40³ × 200-step member of the C2 family (same layers,
+                      source and receiver layout) against the anomaly model's
+                      data, for the gradient error the bench line reports.
 
 The C4 and C5 gathers are stored as float32 roundings of the FP64 results
 (relative rounding ≤ 6e-8, far below the 1e-5 parity bar), to keep the files
@@ -60,6 +64,28 @@ def c5():
     np.savez_compressed(os.path.join(HERE, "c5_two_shots.npz"), gathers=g.astype(np.float32), nt=1024,
                         src=wl.src, m6=wl.m6)
     return g
+
+
+def c2grad(n=40, nt=200):
+    wl, wt = workloads.c2(n=n, nt=nt), workloads.c2(n=n, nt=nt, anomaly=True)
+    cp = O.Cpml.from_dict({k: v[0] for k, v in wl.cpml().items()})
+    c = wl.dt / wl.dx
+    w = wl.wavelets()[0]
+    src = tuple(wl.src[0])
+    lt, mt = wt.lame()
+    obs, _ = O.forward(O.Medium(lt, mt, wt.rho), cp, c, src, wl.m6[0], w, wl.rec[0], nt,
+                       threads=O.default_threads())
+    lam, mu = wl.lame()
+    med = O.Medium(lam, mu, wl.rho)
+    hist = []
+    g, _ = O.forward(med, cp, c, src, wl.m6[0], w, wl.rec[0], nt, threads=O.default_threads(),
+                     history=hist)
+    r, J = O.misfit(g, obs)
+    ref = O.exact_adjoint(med, cp, c, src, w, wl.rec[0], nt, r, hist)
+    np.savez_compressed(os.path.join(HERE, "c2_gradient_40.npz"), lam=ref["lam"].astype(np.float32),
+                        mu=ref["mu"].astype(np.float32), rho=ref["rho"].astype(np.float32),
+                        m6=np.asarray(ref["m6"], np.float64), J=np.float64(J), n=n, nt=nt)
+    return ref["lam"]
 
 
 def main(which):
