@@ -45,12 +45,14 @@ class Survey:
         return sh
 
     def shot_bytes(self):
-        """HBM one shot needs for a checkpointed gradient (snapshots, one
-        segment, adjoint state, accumulators, gathers)."""
+        """HBM one shot needs for a checkpointed gradient (full-state snapshots,
+        one segment of tape states and their shared stresses, adjoint state,
+        accumulators, gathers)."""
         probe = Propagator(self.grid, *self._dummy_medium(), self.batch([0], self.wl.m6))
         K = probe.checkpoint_interval()
-        st = probe.state_bytes()
-        return (2 * K + 4) * st + 8 * 4 * self.grid.N + 4 * 4 * 3 * self.wl.ntmax * self.wl.rec.shape[1]
+        sf, sb = probe.full_state_bytes(), probe.state_bytes()
+        return ((K + 4) * sf + (K + 1) * sb + probe._tau_bytes() + 8 * 4 * self.grid.N
+                + 4 * 4 * 3 * self.wl.ntmax * self.wl.rec.shape[1])
 
     def _dummy_medium(self):
         z = torch.ones(self.grid.shape, device=self.dev)

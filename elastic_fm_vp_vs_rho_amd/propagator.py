@@ -20,6 +20,7 @@ from ._capi import EfmDims, EfmMedium, EfmShots, EfmState, check
 
 H_STENCIL = 2
 FIELDS = ("vx", "vy", "vz", "txx", "tyy", "tzz", "txy", "txz", "tyz")
+FIELD_PAD = 1088   # floats between fields of one allocation (keeps equal cells off power-of-two strides)
 
 
 def _ptr(t):
@@ -160,17 +161,17 @@ class Wavefield:
         ny, nz, nx = grid.shape
         dev = torch.device(device)
         if psi_only_of is None:
-            per = S * grid.N
-            pad = 1088
-            buf = torch.zeros(9 * (per + pad), dtype=torch.float32, device=dev)
-            self.fields = torch.as_strided(buf, (9, S, ny, nz, nx),
-                                           (per + pad, grid.N, nz * nx, nx, 1))
+            self.fields = _padded_fields(9, grid, S, dev)
         else:
             self.fields = psi_only_of.fields
         self.psi_x = torch.zeros((S, 6, ny, nz, Lx), dtype=torch.float32, device=dev)
         self.psi_y = torch.zeros((S, 6, Ly, nz, nx), dtype=torch.float32, device=dev)
         self.psi_z = torch.zeros((S, 6, ny, Lz, nx), dtype=torch.float32, device=dev)
         self._s = None
+
+    @property
+    def device(self):
+        return self.fields.device
 
     def __getitem__(self, name):
         return self.fields[FIELDS.index(name)]
@@ -196,6 +197,58 @@ class Wavefield:
 
     def nbytes(self):
         return sum(t.numel() * 4 for t in (self.fields, self.psi_x, self.psi_y, self.psi_z))
+
+
+def _padded_fields(n, grid, S, device):
+    """(n, S, ny, nz, nx) float32 zeros in one allocation with the padded field
+    stride of Wavefield."""
+    ny, nz, nx = grid.shape
+    per = S * grid.N
+    buf = torch.zeros(n * (per + FIELD_PAD), dtype=torch.float32, device=device)
+    return torch.as_strided(buf, (n, S, ny, nz, nx), (per + FIELD_PAD, grid.N, nz * nx, nx, 1))
+
+
+class TapeState:
+    """A forward state kept for the exact adjoint: its own velocities and CPML ψ
+    slabs, and the stresses of one buffer that every state of the tape shares.
+    The adjoint reads the stored states' v (the ρ products come from the
+    velocity change, the λ/μ products from the strains) and ψ, never their
+    stresses, so one live copy of τ carries the forward steps (the stress sweep
+    updates each cell in place).  A tape state takes 3 volumes + ψ instead of 9
+    + ψ.  `tau` is the shared (6, S, ny, nz, nx) buffer."""
+
+    def __init__(self, grid: Grid, S: int, device, tau):
+        self.grid, self.S = grid, S
+        Lx, Ly, Lz = grid.slab_lengths
+        ny, nz, nx = grid.shape
+        dev = torch.device(device)
+        self.v = _padded_fields(3, grid, S, dev)
+        self.tau = tau
+        self.psi_x = torch.zeros((S, 6, ny, nz, Lx), dtype=torch.float32, device=dev)
+        self.psi_y = torch.zeros((S, 6, Ly, nz, nx), dtype=torch.float32, device=dev)
+        self.psi_z = torch.zeros((S, 6, ny, Lz, nx), dtype=torch.float32, device=dev)
+        self._s = None
+
+    @property
+    def device(self):
+        return self.v.device
+
+    def struct(self):
+        if self._s is None:
+            self._s = EfmState(*[_ptr(self.v[k]) for k in range(3)], *[_ptr(self.tau[k]) for k in range(6)],
+                               _ptr(self.psi_x), _ptr(self.psi_y), _ptr(self.psi_z))
+        return self._s
+
+    def zero_(self):
+        """Zero this state and the shared stresses (the initial state of a run)."""
+        self.v.zero_(); self.tau.zero_(); self.psi_x.zero_(); self.psi_y.zero_(); self.psi_z.zero_()
+        return self
+
+    def copy_(self, other: "Wavefield"):
+        """Load a full state: its v and ψ here, its stresses into the shared buffer."""
+        self.v.copy_(other.fields[:3]); self.tau.copy_(other.fields[3:])
+        self.psi_x.copy_(other.psi_x); self.psi_y.copy_(other.psi_y); self.psi_z.copy_(other.psi_z)
+        return self
 
 
 def lame(vp, vs, rho):
@@ -254,7 +307,7 @@ class Propagator:
         self.lam, self.mu, self.rho = lam, mu, rho
         self.lib = _capi.lib()
         if pool is not None:
-            if any(w.grid != grid or w.S != shots.S or w.fields.device != lam.device for w in pool[:1]):
+            if any(w.grid != grid or w.S != shots.S or w.device != lam.device for w in pool[:1]):
                 raise ValueError("the state pool belongs to another grid, shot count or device")
             self._pool = pool
 
@@ -337,11 +390,24 @@ class Propagator:
     def checkpoint_interval(self):
         return max(1, int(math.ceil(math.sqrt(self.shots.ntmax))))
 
-    def state_bytes(self):
+    def _psi_bytes(self):
         g, S = self.grid, self.shots.S
         Lx, Ly, Lz = g.slab_lengths
         ny, nz, nx = g.shape
-        return 4 * (9 * (S * g.N + 1088) + 6 * S * (ny * nz * Lx + Ly * nz * nx + ny * Lz * nx))
+        return 4 * 6 * S * (ny * nz * Lx + Ly * nz * nx + ny * Lz * nx)
+
+    def full_state_bytes(self):
+        """Bytes of a full state (Wavefield: 9 volumes + ψ), as a checkpoint
+        snapshot takes."""
+        return 4 * 9 * (self.shots.S * self.grid.N + FIELD_PAD) + self._psi_bytes()
+
+    def state_bytes(self):
+        """Bytes of one kept forward state of a tape (TapeState: 3 volumes + ψ;
+        the tape's stresses are one shared buffer of 6 volumes)."""
+        return 4 * 3 * (self.shots.S * self.grid.N + FIELD_PAD) + self._psi_bytes()
+
+    def _tau_bytes(self):
+        return 4 * 6 * (self.shots.S * self.grid.N + FIELD_PAD)
 
     def choose_strategy(self, budget=None):
         """'store' keeps every forward state (no re-forward; (nt+1) states of
@@ -352,36 +418,41 @@ class Propagator:
 
     def plan(self, budget=None):
         """(strategy, m): m = number of final steps whose states are kept."""
-        nt, sb = self.shots.ntmax, self.state_bytes()
+        nt, sb, sf, tb = self.shots.ntmax, self.state_bytes(), self.full_state_bytes(), self._tau_bytes()
         have = getattr(self, "_pool", [])
         if len(have) >= nt + 1:
             return "store", nt
         if budget is None:
-            # states already in the pool are reused, so they count as available too
-            budget = 0.85 * (available_bytes(self.lam.device) + len(have) * sb)
-        if (nt + 1) * sb <= budget:
+            # states already in the pool (and their shared stresses) are reused, so they count
+            # as available too
+            budget = 0.85 * (available_bytes(self.lam.device) + len(have) * sb + (tb if have else 0))
+        if (nt + 1) * sb + tb <= budget:
             return "store", nt
-        # hybrid: (m + 1) kept states + the snapshots of the first nt - m steps
+        # hybrid: (m + 1) kept states + the full snapshots of the first nt - m steps
         # (the segment buffers reuse the kept states once their sweep is done),
-        # with room left for the adjoint state, ψ ping-pong and accumulators
-        fit = int(budget // sb) - 4
-        m = fit
+        # with room left for the adjoint state, ψ ping-pong and accumulators (4 full states)
+        room = budget - tb - 4 * sf
+        m = int(room // sb)
         while m > 0:
             ncp = nt - m
             K = max(1, math.isqrt(max(ncp - 1, 0)) + 1)
-            if m + 1 + -(-ncp // K) <= fit and m >= K:
+            if (m + 1) * sb + -(-ncp // K) * sf <= room and m >= K:
                 return "hybrid", m
             m -= max(1, m // 64)
         return "checkpoint", 0
 
     def _states(self, n):
-        """At least n zeroed wavefields, allocated once per propagator and reused
-        (kernels write only interior cells, so the zero borders survive)."""
+        """At least n tape states (zeroed when allocated, sharing one stress
+        buffer), allocated once per pool and reused (kernels write only interior
+        cells, so the zero borders survive)."""
         pool = getattr(self, "_pool", None)
         if pool is None:
             pool = self._pool = []
+        tau = pool[0].tau if pool else None
         while len(pool) < n:
-            pool.append(self.new_wavefield())
+            if tau is None:
+                tau = _padded_fields(6, self.grid, self.shots.S, self.lam.device)
+            pool.append(TapeState(self.grid, self.shots.S, self.lam.device, tau))
         return pool[:n]
 
     def release_states(self):

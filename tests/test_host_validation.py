@@ -82,22 +82,26 @@ def test_tape_plan_fits_the_budget():
                    [[(5, 5, 5)]], cp, device="cpu", fuse_receivers=False)
     z = torch.ones((n, n, n))
     prop = Propagator(grid, z, z, z, sh)
-    sb = prop.state_bytes()
-    assert prop.plan((nt + 1) * sb) == ("store", nt)
-    for frac in (0.9, 0.5, 0.2, 0.12):
-        budget = frac * (nt + 1) * sb
+    sb, sf, tb = prop.state_bytes(), prop.full_state_bytes(), prop._tau_bytes()
+    # a tape state holds v (3 volumes) and ψ; the stresses are one shared buffer of 6 volumes
+    assert sf - sb == tb
+    assert prop.plan((nt + 1) * sb + tb) == ("store", nt)
+    assert prop.plan((nt + 1) * sb + tb - 1)[0] == "hybrid"
+    for frac in (0.9, 0.6, 0.4, 0.25):
+        budget = frac * ((nt + 1) * sb + tb)
         strategy, m = prop.plan(budget)
         assert strategy == "hybrid", (frac, strategy)
         ncp = nt - m
         K = math.isqrt(ncp - 1) + 1
         assert 0 < m < nt and m >= K
-        assert (m + 1 + -(-ncp // K)) * sb <= budget
+        # kept tape states + full snapshots + shared stresses + 4 full states of adjoint room
+        assert (m + 1) * sb + -(-ncp // K) * sf + tb + 4 * sf <= budget
         # close to the best: keeping 2 % more would not fit
         m2 = int(m * 1.02) + 2
         ncp2 = nt - m2
         K2 = math.isqrt(max(ncp2 - 1, 0)) + 1
-        assert (m2 + 1 + -(-ncp2 // K2) + 4) * sb > budget
-    assert prop.plan(10 * sb)[0] == "checkpoint"
+        assert (m2 + 1) * sb + -(-ncp2 // K2) * sf + tb + 4 * sf > budget
+    assert prop.plan(10 * sf)[0] == "checkpoint"
 
 
 def test_fwi_sub_batches_are_equal_and_cover_every_shot():
