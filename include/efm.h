@@ -165,7 +165,9 @@ int efm_forward_states(const EfmDims *d, const EfmMedium *m, const EfmShots *sh,
 int efm_derived_medium(const EfmDims *d, const EfmMedium *m, float *mbar, float *bbar,
                        void *stream);
 /* Reverse sweep of steps n = n1-1 down to n0 (n0 < n1).  fwd[k] is the
- * forward pre-step state S^(n0+k), k = 0 .. n1-n0 (so fwd[n1-n0] is S^n1).
+ * forward pre-step state S^(n0+k), k = 0 .. n1-n0 (so fwd[n1-n0] is S^n1); only
+ * its velocities and ψ slabs are read (its stress pointers must be valid but may
+ * all share one buffer).
  * adj/adj2 are the adjoint state and a second ψ buffer (ping-pong; on return
  * the live ψ adjoint is in adj).  resid is ∂J/∂gathers (S, 3, ntmax, nrec).
  * acc (S·8 components of stride N + EFM_PAD, zeroed by the caller) accumulates
