@@ -58,11 +58,7 @@ int set_tile(int kernel, Tile t) {
 // counted in 64 bits).
 static long long sweep_items(const Geo &g, const Tile &t) {
     const long long per = (long long)t.bx * t.bz;
-    if (t.map > 0) {
-        const long long B = sweep_tiles(g, t.bx, t.bz, t.ny, t.map);
-        const int k = map_xcds(t.map);
-        return k * ((B + k - 1) / k) * per;
-    }
+    if (t.map > 0) return xcd_blocks(g, t) * per;
     const long long nbx = (g.nx + t.bx - 1) / t.bx, nbz = (g.nz + t.bz - 1) / t.bz;
     const long long nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
     return nbx * nbz * nyc * g.S * per;

@@ -179,21 +179,28 @@ inline long long sweep_tiles(const Geo &g, int bx, int bz, int ny, int map) {
     return nbx * nbz * nyc * g.S;
 }
 
+// Blocks of an XCD-aware sweep: each shot's tiles are dealt over the k XCDs in
+// runs of ceil(tiles/k) (see tile_coords), so a shot takes k·ceil(tiles/k).
+inline long long xcd_blocks(const Geo &g, const Tile &t) {
+    const long long per_shot = sweep_tiles(g, t.bx, t.bz, t.ny, t.map) / g.S;
+    const int k = map_xcds(t.map);
+    return k * ((per_shot + k - 1) / k) * g.S;
+}
+
 inline dim3 sweep_grid(const Geo &g, const Tile &t) {
-    if (t.map > 0) {
-        const long long B = sweep_tiles(g, t.bx, t.bz, t.ny, t.map);
-        const int k = map_xcds(t.map);
-        return dim3((unsigned)(k * ((B + k - 1) / k)));
-    }
+    if (t.map > 0) return dim3((unsigned)xcd_blocks(g, t));
     const int nyc = (g.ny - 2 * H + t.ny - 1) / t.ny;
     return dim3((g.nx + t.bx - 1) / t.bx, (g.nz + t.bz - 1) / t.bz, g.S * nyc);
 }
 
 // Tile coordinates of the calling block.  MAP = 0: the 3-D grid as launched.
 // MAP = k: workgroups are dealt to the k XCDs round-robin (w mod k), each with
-// its own L2, so block w is given tile k_w·per + w/k: every XCD walks a
-// contiguous run of tiles (x fastest, then z, then y), and the ±z rows and ±y
-// planes a tile reads were fetched into the same L2 by its neighbours.
+// its own L2, so block w is given a tile of its XCD's contiguous run: every XCD
+// walks a run of a shot's tiles (x fastest, then z, then y), and the ±z rows and
+// ±y planes a tile reads were fetched into the same L2 by its neighbours.  The
+// runs are per shot (XCD k takes the k-th run of shot 0, then of shot 1, ...), so
+// all XCDs work on the same shot at a time and a batch's sweep keeps one shot's
+// working set in the Infinity Cache, as a one-shot sweep does.
 // MAP = 16: the same XCD order over blocks of BX·BZ consecutive cells of the
 // interior (x, z) plane numbered x fastest with rows wrapped, so that every
 // 64-lane wave is full whatever nx is (with nx - 4 not a multiple of 64 the
@@ -208,15 +215,15 @@ __device__ __forceinline__ bool tile_coords(const Geo &g, int &x, int &z, int &s
         const int nxi = g.nx - 2 * H;
         const int Q = nxi * (g.nz - 2 * H);
         const int nqb = (Q + BX * BZ - 1) / (BX * BZ);
-        const int B = nqb * nyc * g.S;
-        const int per = (B + 7) / 8;
+        const int per_shot = nqb * nyc;
+        const int run = (per_shot + 7) / 8;
         const int w = blockIdx.x;
-        const int gi = (w % 8) * per + w / 8;
-        if (gi >= B) return false;
+        const int j = w / 8;
+        s = j / run;
+        const int gi = (w % 8) * run + (j - s * run);
+        if (s >= g.S || gi >= per_shot) return false;
         const int qb = gi % nqb;
-        const int r = gi / nqb;
-        yb = r % nyc;
-        s = r / nyc;
+        yb = gi / nqb;
         const int q = qb * (BX * BZ) + threadIdx.y * BX + threadIdx.x;
         if (q >= Q) {
             x = z = -1;
@@ -235,17 +242,18 @@ __device__ __forceinline__ bool tile_coords(const Geo &g, int &x, int &z, int &s
         return true;
     }
     const int nbx = (g.nx + BX - 1) / BX, nbz = (g.nz + BZ - 1) / BZ;
-    const int B = nbx * nbz * nyc * g.S;
-    const int per = (B + MAP - 1) / MAP;
+    constexpr int K = MAP > 0 ? MAP : 1;
+    const int per_shot = nbx * nbz * nyc;
+    const int run = (per_shot + K - 1) / K;
     const int w = blockIdx.x;
-    const int gi = (w % MAP) * per + w / MAP;
-    if (gi >= B) return false;
+    const int j = w / K;
+    s = j / run;
+    const int gi = (w % K) * run + (j - s * run);
+    if (s >= g.S || gi >= per_shot) return false;
     const int xb = gi % nbx;
-    int r = gi / nbx;
+    const int r = gi / nbx;
     const int zb = r % nbz;
-    r /= nbz;
-    yb = r % nyc;
-    s = r / nyc;
+    yb = r / nbz;
     x = xb * BX + threadIdx.x;
     z = zb * BZ + threadIdx.y;
     return true;
