@@ -246,6 +246,8 @@ class TapeState:
 
     def copy_(self, other: "Wavefield"):
         """Load a full state: its v and ψ here, its stresses into the shared buffer."""
+        if not isinstance(other, Wavefield):
+            raise TypeError("a tape state loads a full Wavefield (a snapshot or a live state)")
         self.v.copy_(other.fields[:3]); self.tau.copy_(other.fields[3:])
         self.psi_x.copy_(other.psi_x); self.psi_y.copy_(other.psi_y); self.psi_z.copy_(other.psi_z)
         return self
