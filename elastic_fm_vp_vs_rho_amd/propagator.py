@@ -483,11 +483,28 @@ class Propagator:
     def record(self, K=None, strategy="auto", m=None):
         """Forward run that keeps what the adjoint needs.  Returns a tape dict.
         K: checkpoint interval; m: steps kept by a hybrid tape (default: as many
-        as fit)."""
-        if strategy == "auto":
-            strategy, m_fit = self.plan()
-            if strategy == "hybrid" and m is None:
-                m = m_fit
+        as fit).  With strategy "auto", a tape the plan sized but the device
+        cannot hold after all (memory held elsewhere in the process, allocator
+        fragmentation) falls back to √nt checkpoints instead of failing."""
+        if strategy != "auto":
+            return self._record(K, strategy, m)
+        planned, m_fit = self.plan()
+        if planned == "hybrid" and m is None:
+            m = m_fit
+        held = len(getattr(self, "_pool", None) or [])
+        try:
+            return self._record(K, planned, m)
+        except torch.OutOfMemoryError:
+            if planned == "checkpoint":
+                raise
+            pool = getattr(self, "_pool", None)
+            if pool is not None:
+                del pool[held:]                  # give back the states this call added
+            if torch.cuda.is_initialized():
+                torch.cuda.empty_cache()
+            return self._record(K, "checkpoint", None)
+
+    def _record(self, K, strategy, m):
         nt = self.shots.ntmax
         g = self._kept("gathers", self.new_gathers)
         if self.graphs:

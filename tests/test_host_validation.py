@@ -151,3 +151,36 @@ def test_tape_states_share_one_stress_buffer():
     states[0].zero_()
     assert float(states[2].tau.abs().max()) == 0.0
     assert prop.state_bytes() + prop._tau_bytes() == prop.full_state_bytes()
+
+
+def test_record_falls_back_to_checkpoints_when_the_tape_does_not_fit():
+    """With strategy "auto", a store or hybrid tape the plan sized but the
+    device cannot allocate (memory held elsewhere, fragmentation) gives way to
+    √nt checkpoints; the states the failed attempt added to the pool are given
+    back.  (Control flow only: the time loops are stubbed.)"""
+    import numpy as np
+    import torch
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch
+    n, nt = 12, 30
+    grid = Grid(n, n, n, 10.0, 1e-3, (2,) * 6)
+    cp = {k: np.zeros((1, n), np.float32) for k in ("ax", "bx", "ay", "by", "az", "bz")}
+    sh = ShotBatch(grid, [(6, 6, 6)], [np.ones(6)], np.zeros((1, nt), np.float32), [nt], [[(4, 4, 4)]], cp,
+                   device="cpu", fuse_receivers=False)
+    z = torch.ones((n, n, n))
+    pool = []
+    prop = Propagator(grid, z, z, z, sh, pool=pool)
+    prop.plan = lambda budget=None: ("store", nt)
+    prop.forward = lambda st, g, it0=0, nsteps=None, **kw: (g, st)
+    prop._forward_states = lambda arr, g, it0, nsteps: None
+    real_states = prop._states
+
+    def short_of_memory(k):
+        real_states(min(k, 5))                   # some states fit, then the allocator gives up
+        raise torch.OutOfMemoryError("simulated")
+
+    prop._states = short_of_memory
+    tape = prop.record()
+    assert tape["strategy"] == "checkpoint" and tape["K"] == prop.checkpoint_interval()
+    assert len(pool) == 0
+    with pytest.raises(torch.OutOfMemoryError):
+        prop.record(strategy="store")            # an explicit strategy is not second-guessed
