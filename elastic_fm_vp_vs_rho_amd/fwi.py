@@ -265,7 +265,7 @@ def plain_driver(wl, obs, device):
 
 
 def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, log=None,
-                  host_loop=True):
+                  host_loop=True, host_iters=0, step=None):
     """FWI on a 256×256×128 survey: the plain single-shot path (plain_driver,
     one earthquake at a time, host-driven, the reference's iteration rules),
     with the reference's imaging conditions and with the exact gradient, against
@@ -276,7 +276,9 @@ def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, 
     lowered the misfit at all).  host_loop: also time one gradient of the plain
     path driven the way the reference drives it (one host call per time step,
     each reverse iteration's three images copied to the host and summed there,
-    M:1823, 1939-2054)."""
+    M:1823, 1939-2054); host_iters > 0 also runs that many iterations of the
+    host-driven plain inversion (its wall time to its final misfit).  step:
+    the batched inversion's first-step sizes (invert's `step`; default its own)."""
     dev = torch.device("cuda:0")
     wt = workloads.c4(nx=nx, ny=ny, nz=nz, S=S, nt=nt, anomalies=True)
     w0 = workloads.c4(nx=nx, ny=ny, nz=nz, S=S, nt=nt, anomalies=False)
@@ -331,6 +333,15 @@ def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, 
             "s_per_gradient": t_host}
         if log:
             log(f"[compare] plain (reference, host-driven): {t_host:.2f} s per gradient")
+        if host_iters > 0:
+            cost, t_inv = timed(lambda: e3.fwi_Vp_Vs_Rho(gathers, listTM, host_iters, lbfgsLayers=7,
+                                                         ite_cte=2, gradient="reference"))
+            J_end = float(e3._misfit_all(gathers, listTM))
+            out["plain_host_driven"].update(iterations=host_iters, wall_s=t_inv,
+                                            J_history=[float(c) for c in cost], J_final=J_end)
+            if log:
+                log(f"[compare] plain (reference, host-driven): {host_iters} iterations in {t_inv:.1f} s, "
+                    f"J {J0:.4e} -> {J_end:.4e}")
         e3.release_states()
         del e3
         torch.cuda.empty_cache()
@@ -346,12 +357,14 @@ def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, 
 
     torch.cuda.synchronize()
     t1[0] = time.perf_counter()
+    kw = {} if step is None else {"step": tuple(step)}
     hist, _ = invert(survey, f(w0.vp), f(w0.vs), f(w0.rho), w0.m6.copy(), obs, iters, batch, log=log,
-                     callback=mark)
+                     callback=mark, **kw)
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t1[0]
     out["batched"] = {"path": "fwi.invert: joint L-BFGS on Vp, Vs, rho and MT, exact gradient, "
                               f"{batch} shot(s) per sub-batch",
+                      "step": list(step) if step is not None else "default",
                       "s_per_gradient": t_bgrad, "iterations": len(hist) - 1, "wall_s": t_batched,
                       "J_history": [float(J) for J in hist]}
     for grad, pl in out["plain"].items():
@@ -361,8 +374,13 @@ def compare_plain(nx=256, ny=256, nz=128, S=8, nt=600, plain_iters=5, iters=12, 
         pl["speedup_to_equal_misfit"] = pl["wall_s"] / reach[2] if reach else None
         pl["plain_over_batched_s_per_gradient"] = pl["s_per_gradient"] / t_bgrad
     if "plain_host_driven" in out:
-        out["plain_host_driven"]["plain_over_batched_s_per_gradient"] = (
-            out["plain_host_driven"]["s_per_gradient"] / t_bgrad)
+        ph = out["plain_host_driven"]
+        ph["plain_over_batched_s_per_gradient"] = ph["s_per_gradient"] / t_bgrad
+        if "J_final" in ph:
+            target = ph["J_final"]
+            reach = next((m for m in marks if m[1] <= target), None) if target < J0 else None
+            ph["batched_time_to_this_misfit_s"] = reach[2] if reach else None
+            ph["speedup_to_equal_misfit"] = ph["wall_s"] / reach[2] if reach else None
     return out
 
 
@@ -384,13 +402,18 @@ def main(argv=None):
     ap.add_argument("--plain-iters", type=int, default=5)
     ap.add_argument("--no-host-loop", action="store_true",
                     help="skip timing the plain gradient driven one step at a time from the host")
+    ap.add_argument("--host-iters", type=int, default=0,
+                    help="also run this many iterations of the host-driven plain inversion")
+    ap.add_argument("--step", default=None,
+                    help="first-step sizes of the batched inversion: Vp,Vs,rho,MT (default: invert's own)")
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
     if args.compare_plain:
         print(json.dumps(compare_plain(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt,
                                        plain_iters=args.plain_iters, iters=args.iters, log=log,
-                                       host_loop=not args.no_host_loop)))
+                                       host_loop=not args.no_host_loop, host_iters=args.host_iters,
+                                       step=[float(v) for v in args.step.split(",")] if args.step else None)))
         return
     wt = workloads.c4(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt, anomalies=True)
     w0 = workloads.c4(nx=args.nx, ny=args.nx, nz=args.nz, S=args.shots, nt=args.nt, anomalies=False)
