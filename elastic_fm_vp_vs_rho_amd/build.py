@@ -22,6 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libefm.so")
+HOST_EXAMPLE = os.path.join(HERE, "efm_host_example")
 SOURCES = ("forward.hip", "adjoint.hip", "reference.hip", "capi.hip", "initnet.hip", "conditioning.hip", "plan.hip")
 ARCH = os.environ.get("EFM_OFFLOAD_ARCH", "gfx950")
 
@@ -63,7 +64,24 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host_example()
     return LIB
+
+
+def build_host_example():
+    """The C host example (csrc/host_example.c): a program without Python that
+    links ROCm's HIP runtime and libefm.so (found next to it)."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    tmp = HOST_EXAMPLE + ".tmp"
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE,
+           "-I", os.path.join(rocm, "include"), os.path.join(CSRC, "host_example.c"), "-o", tmp,
+           "-L", HERE, "-l:libefm.so", "-Wl,-rpath,$ORIGIN",
+           "-L", os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host example build failed:\n{r.stderr}")
+    os.replace(tmp, HOST_EXAMPLE)
+    return HOST_EXAMPLE
 
 
 if __name__ == "__main__":

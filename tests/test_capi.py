@@ -35,6 +35,16 @@ def test_every_declared_symbol_is_exported_and_bound(L):
     assert set(names) == set(_capi.exported_symbols())
 
 
+def test_c_host_example_is_built_and_links():
+    """The C host (csrc/host_example.c) links ROCm's HIP runtime and libefm.so
+    and starts without Python; run without arguments it prints its usage."""
+    import subprocess
+    exe = os.path.join(ROOT, "elastic_fm_vp_vs_rho_amd", "efm_host_example")
+    assert os.path.exists(exe), "efm_host_example is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "usage" in r.stderr, (r.returncode, r.stderr)
+
+
 def test_single_hip_runtime_mapped(L):
     from elastic_fm_vp_vs_rho_amd import _capi
     rts = _capi._hip_runtimes_mapped()
