@@ -63,6 +63,12 @@ static float *zeros(size_t n) {
     return d;
 }
 
+static void free_state(EfmState *s) {
+    float *v[12] = {s->vx, s->vy, s->vz, s->txx, s->tyy, s->tzz, s->txy, s->txz, s->tyz,
+                    s->psi_x, s->psi_y, s->psi_z};
+    for (int k = 0; k < 12; ++k) HIPCHECK(hipFree(v[k]));
+}
+
 static EfmState new_state(size_t N, size_t npx, size_t npy, size_t npz) {
     EfmState s;
     float **v[9] = {&s.vx, &s.vy, &s.vz, &s.txx, &s.tyy, &s.tzz, &s.txy, &s.txz, &s.tyz};
@@ -159,6 +165,15 @@ int main(int argc, char **argv) {
     fwrite(hd, 8, 6, o);
     fclose(o);
     free(hb);
+
+    for (int k = 0; k <= nt; ++k) free_state(&states[k]);
+    free_state(&adj);
+    free(states);
+    free(fwd);
+    const void *bufs[] = {m.lam, m.mu, m.rho, sh.ax, sh.bx, sh.ay, sh.by, sh.az, sh.bz, sh.src,
+                          sh.m6, sh.wavelet, sh.rec, ntd, obs, gathers, resid, glam, gmu, grho, J, gm6};
+    for (size_t k = 0; k < sizeof bufs / sizeof bufs[0]; ++k) HIPCHECK(hipFree((void *)bufs[k]));
+    HIPCHECK(hipStreamDestroy(st));
     printf("efm_host_example: %dx%dx%d, %d steps, %d receivers: done\n", d.nx, d.ny, d.nz, nt, nrec);
     return 0;
 }
