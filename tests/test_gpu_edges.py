@@ -1,6 +1,7 @@
 """Edge cases on the GPU: no receivers at all, unused receiver slots, record
-lengths of one step next to longer ones, and the smallest grid with a single
-interior cell and no absorbing layer — each against the FP64 oracle."""
+lengths of one step next to longer ones, the smallest grid with a single
+interior cell and no absorbing layer, receivers off the interior, and a
+different absorbing-layer width on every face — each against the FP64 oracle."""
 import numpy as np
 import pytest
 import torch
@@ -105,4 +106,43 @@ def test_gradient_with_receivers_off_the_interior(gpu_lib):
     for name, got in (("lam", glam), ("mu", gmu), ("rho", grho)):
         e = rel_l2(got.double().cpu().numpy(), ref[name])
         assert e < 1e-4, (name, e)
+    assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < 1e-4
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+def test_different_cpml_width_on_every_face(gpu_lib, fuse):
+    """A different absorbing-layer width on each face (x 5 / 3, y 3 / 5, z 0 /
+    4; the C ABI's ψ slabs have one extent per face): seismograms and the exact
+    gradient against the oracle on the same profiles, which are zero outside
+    each face's layer (the oracle keeps full-volume ψ and full-length profiles,
+    so the slab extents are the device's alone)."""
+    from dataclasses import replace
+    from elastic_fm_vp_vs_rho_amd.propagator import Grid, Propagator, ShotBatch
+    p = tiny(n=(30, 26, 22), nt=110, W=5, seed=51, nrec=8, free_top=True)
+    widths = (5, 3, 3, 5, 0, 4)
+    cp = {k: np.array(v, np.float32) for k, v in p.cp.items()}
+    for keys, n, (lo, hi) in ((("ax", "bx"), p.nx, widths[0:2]), (("ay", "by"), p.ny, widths[2:4]),
+                              (("az", "bz"), p.nz, widths[4:6])):
+        keep = np.zeros(n, bool)
+        keep[:lo] = True
+        keep[n - hi:] = True
+        for k in keys:
+            cp[k][..., ~keep] = 0
+    q = replace(p, cp=cp)
+    assert np.any(cp["ax"][..., 2:5]) and not np.any(cp["ax"][..., q.nx - 5:q.nx - 3])
+    grid = Grid(q.nx, q.ny, q.nz, q.dx, q.dt, widths)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+    prop = Propagator(grid, f(q.lam), f(q.mu), f(q.rho),
+                      ShotBatch(grid, [q.src], [q.m6], [q.wavelet], [q.nt], [q.rec], q.cp,
+                                fuse_receivers=fuse))
+    obs = run_forward(q, lam=q.lam * 1.02, rho=q.rho * 0.99)[0].astype(np.float32)
+    hist = []
+    g_ref, _ = run_forward(q, history=hist)
+    r, _ = O.misfit(g_ref, obs)
+    ref = O.exact_adjoint(q.medium(), q.cpml(), q.c, q.src, q.wavelet, q.rec, q.nt, r, hist)
+    print(f"widths {widths}, fuse {fuse}", flush=True)
+    J, gg, glam, gmu, grho, gm6 = prop.gradient(f(obs[None]), K=13)
+    assert rel_l2(gg[0].double().cpu().numpy(), g_ref) < 1e-5
+    for name, got in (("lam", glam), ("mu", gmu), ("rho", grho)):
+        assert rel_l2(got.double().cpu().numpy(), ref[name]) < 1e-4, name
     assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < 1e-4
