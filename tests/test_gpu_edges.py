@@ -1,8 +1,9 @@
 """Edge cases on the GPU: no receivers at all, unused receiver slots, record
 lengths of one step next to longer ones, the smallest grid with a single
 interior cell and no absorbing layer, receivers off the interior, receivers
-sharing a cell or sitting on the source, and a different absorbing-layer width
-on every face — each against the FP64 oracle."""
+sharing a cell or sitting on the source, a different absorbing-layer width on
+every face, and sources inside the layers of shots with their own profiles —
+each against the FP64 oracle."""
 import numpy as np
 import pytest
 import torch
@@ -179,3 +180,46 @@ def test_receivers_sharing_a_cell_and_on_the_source(gpu_lib, fuse):
     for name, t in (("lam", glam), ("mu", gmu), ("rho", grho)):
         assert rel_l2(t.double().cpu().numpy(), ref[name]) < 1e-4, name
     assert rel_l2(gm6[0].cpu().numpy(), ref["m6"]) < 1e-4
+
+
+def test_sources_in_the_layers_with_per_shot_profiles(gpu_lib):
+    """Two shots in one batch, each with its own CPML profiles (Ricker peaks 1.0
+    and 1.6 × the base frequency: the layer's α follows the frequency) and its
+    source inside the absorbing layers (an x-y edge, a y-z edge): each shot's
+    gathers and ∂J/∂M against its own oracle run, the volume gradients against
+    the sum of the two."""
+    from elastic_fm_vp_vs_rho_amd.propagator import Propagator, ShotBatch
+    p = tiny(n=(28, 24, 22), nt=100, W=5, seed=71, nrec=9)
+    f0 = 0.12 / p.dt / 8
+    vmax = float(p.vp.max())
+    shots = []
+    for k, (fq, src) in enumerate(((f0, (3, 4, 11)), (1.6 * f0, (14, 20, 18)))):
+        cp = O.cpml_coefficients(p.W, fq, p.dt, p.dx, vmax, p.nx, p.ny, p.nz, free_top=False)
+        shots.append(dict(cp=cp, src=src, wav=O.ricker(fq, p.dt, p.nt).astype(np.float32),
+                          m6=np.random.default_rng(k).uniform(-1, 1, 6)))
+    assert not np.allclose(shots[0]["cp"]["ax"], shots[1]["cp"]["ax"])
+    grid = grid_of(p)
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+    prof = {k: np.stack([np.asarray(s["cp"][k], np.float32) for s in shots]) for k in shots[0]["cp"]}
+    sh = ShotBatch(grid, [s["src"] for s in shots], [s["m6"] for s in shots], [s["wav"] for s in shots],
+                   [p.nt] * 2, [p.rec] * 2, prof)
+    prop = Propagator(grid, f(p.lam), f(p.mu), f(p.rho), sh)
+    med_true = O.Medium(p.lam * 1.02, p.mu * 0.98, p.rho)
+    obs = np.zeros((2, 3, p.nt, len(p.rec)), np.float32)
+    refs, gref = [], {"lam": 0.0, "mu": 0.0, "rho": 0.0}
+    for k, s in enumerate(shots):
+        cpk = O.Cpml.from_dict(s["cp"])
+        obs[k] = O.forward(med_true, cpk, p.c, s["src"], s["m6"], s["wav"], p.rec, p.nt)[0]
+        hist = []
+        g, _ = O.forward(p.medium(), cpk, p.c, s["src"], s["m6"], s["wav"], p.rec, p.nt, history=hist)
+        r, _ = O.misfit(g, obs[k])
+        a = O.exact_adjoint(p.medium(), cpk, p.c, s["src"], s["wav"], p.rec, p.nt, r, hist)
+        refs.append((g, a["m6"]))
+        for name in gref:
+            gref[name] = gref[name] + a[name]
+    J, gg, glam, gmu, grho, gm6 = prop.gradient(f(obs), K=10)
+    for k, (g, m6) in enumerate(refs):
+        assert rel_l2(gg[k].double().cpu().numpy(), g) < 1e-5, k
+        assert rel_l2(gm6[k].cpu().numpy(), m6) < 1e-4, k
+    for name, t in (("lam", glam), ("mu", gmu), ("rho", grho)):
+        assert rel_l2(t.double().cpu().numpy(), gref[name]) < 1e-4, name
