@@ -111,8 +111,9 @@ def test_exact_gradient_free_top_two_uneven_shots_match_oracle(gpu_lib):
     """Free surface on top; two shots in one batch with different sources (one
     three cells below the surface), MTs and record lengths (120 and 83 steps).
     Each shot's gathers and ∂J/∂M against its own oracle run, the volume
-    gradients against the sum of the two oracle gradients, for the stored and
-    the √-checkpointed tape (the short shot ends inside a segment)."""
+    gradients against the sum of the two oracle gradients, for the stored, the
+    √-checkpointed (the short shot ends inside a segment) and the hybrid tape
+    (the short shot ends inside the checkpointed part)."""
     from elastic_fm_vp_vs_rho_amd.propagator import Propagator, ShotBatch
     from tests.gpu_helpers import grid_of
     p = tiny(n=(30, 20, 18), nt=120, W=4, seed=31, nrec=9, free_top=True, src=(12, 9, 3))
@@ -139,8 +140,10 @@ def test_exact_gradient_free_top_two_uneven_shots_match_oracle(gpu_lib):
                       [s[2] for s in shots], [p.rec] * 2, p.cp)
     prop = Propagator(grid, f(p.lam), f(p.mu), f(p.rho), batch)
     out = {}
-    for strategy in ("store", "checkpoint"):
-        J, gg, glam, gmu, grho, gm6 = prop.gradient(f(obs), K=11, strategy=strategy)
+    for strategy in ("store", "checkpoint", "hybrid"):
+        kw = dict(m=30) if strategy == "hybrid" else dict(K=11)   # hybrid: steps 0-89 checkpointed
+        J, gg, glam, gmu, grho, gm6 = prop.gradient(f(obs), strategy=strategy, **kw)
+        assert prop.last_strategy == strategy
         out[strategy] = (glam, gmu, grho, gm6)
         assert abs(J - J_ref) <= 1e-4 * J_ref, (J, J_ref)
         for s, (_, _, nt) in enumerate(shots):
@@ -150,7 +153,8 @@ def test_exact_gradient_free_top_two_uneven_shots_match_oracle(gpu_lib):
         for name, got in (("lam", glam), ("mu", gmu), ("rho", grho)):
             e = rel_l2(got.double().cpu().numpy(), ref[name])
             assert e < GRAD_TOL, (strategy, name, e)
-    assert all(torch.equal(a, b) for a, b in zip(out["store"], out["checkpoint"]))
+    for other in ("checkpoint", "hybrid"):
+        assert all(torch.equal(a, b) for a, b in zip(out["store"], out[other])), other
 
 
 def test_exact_adjoint_given_the_oracle_residual(gpu_lib):
